@@ -1,0 +1,287 @@
+"""Pins for the CPU oracle (oracle/tn_oracle.py, oracle/models.py).
+
+Each check is chosen so that a plausible mistake in the oracle (a wrong axis
+order, a dropped truncation term, a transposed operand) fails it: known
+answers from the reference README/docstrings, brute force with np.einsum and
+dense state vectors, closed forms, and the invariants of the reference's own
+test file on the reference's own fixture.
+"""
+import numpy as np
+import pytest
+
+from oracle import tn_oracle as o
+from oracle import models as om
+
+
+def T(shape, labels, rng):
+    return o.OTensor(rng.standard_normal(shape), labels)
+
+
+# ---------------------------------------------------------------- axis ops
+def test_move_indices_docstring_known_answer(rng):
+    # tensor.py:408-441
+    t = T((2, 3, 4, 5, 6), ["a", "b", "c", "b", "d"], rng)
+    t.move_indices(["d", "b", "c"], 0, preserve_relative_order=True)
+    assert t.labels == ["b", "c", "b", "d", "a"] and t.shape == (3, 4, 5, 6, 2)
+    t = T((2, 3, 4, 5, 6), ["a", "b", "c", "b", "d"], rng)
+    src = t.data.copy()
+    t.move_indices(["d", "b", "c"], 0, preserve_relative_order=False)
+    assert t.labels == ["d", "b", "b", "c", "a"] and t.shape == (6, 3, 5, 4, 2)
+    assert np.array_equal(t.data, np.transpose(src, (4, 1, 3, 2, 0)))
+
+
+def test_consolidate_indices_fuses_sorted(rng):
+    # tensor.py:340-370: alphabetical, duplicates fused with first occurrence slowest
+    t = T((2, 3, 4, 5), ["z", "a", "z", "b"], rng)
+    src = t.data.copy()
+    t.consolidate_indices()
+    assert t.labels == ["a", "b", "z"] and t.shape == (3, 5, 8)
+    ref = np.transpose(src, (1, 3, 0, 2)).reshape(3, 5, 8)
+    assert np.array_equal(t.data, ref)
+
+
+def test_move_index_is_moveaxis(rng):
+    t = T((2, 3, 4, 5), ["a", "b", "c", "d"], rng)
+    src = t.data.copy()
+    t.move_index("a", 2)
+    assert t.labels == ["b", "c", "a", "d"]
+    assert np.array_equal(t.data, np.moveaxis(src, 0, 2))
+    t.move_index("d", 0)
+    assert t.labels == ["d", "b", "c", "a"]
+
+
+# ---------------------------------------------------------------- contract
+def test_contract_docstring_known_answer(rng):
+    # tensor.py:662-683
+    A = T((2, 2), ["spam", "eggs"], rng)
+    B = T((2, 3, 2, 4), [], rng)
+    assert B.labels == ["i0", "i1", "i2", "i3"]
+    C = o.contract(A, B, "spam", "i2")
+    assert C.shape == (2, 2, 3, 4) and C.labels == ["eggs", "i0", "i1", "i3"]
+    np.testing.assert_allclose(C.data, np.einsum("se,absd->eabd", A.data, B.data), rtol=1e-13)
+    D = o.contract(A, B, ["spam", "eggs"], ["i0", "i2"])
+    assert D.shape == (3, 4) and D.labels == ["i1", "i3"]
+    np.testing.assert_allclose(D.data, np.einsum("se,sbed->bd", A.data, B.data), rtol=1e-13)
+    D2 = A["spam", "eggs"] * B["i0", "i2"]
+    np.testing.assert_array_equal(D2.data, D.data)
+
+
+def test_contract_duplicate_labels_and_slice(rng):
+    # all axes with a listed label are contracted (tensor.py:702-712); index_slice selects
+    A = T((3, 4, 3), ["x", "y", "x"], rng)
+    B = T((3, 5, 3), ["x", "z", "x"], rng)
+    C = o.contract(A, B, "x", "x")
+    np.testing.assert_allclose(C.data, np.einsum("ayb,azb->yz", A.data, B.data), rtol=1e-13)
+    C2 = o.contract(A, B, "x", "x", index_slice1=[-1], index_slice2=[0])
+    assert C2.labels == ["x", "y", "z", "x"]
+    np.testing.assert_allclose(C2.data, np.einsum("ayk,kzb->ayzb", A.data, B.data), rtol=1e-13)
+
+
+def test_contract_dimension_mismatch_raises(rng):
+    A = T((2, 3), ["a", "b"], rng)
+    B = T((4, 3), ["c", "d"], rng)
+    with pytest.raises(ValueError):
+        o.contract(A, B, "a", "c")
+
+
+def test_tensor_product_and_trace(rng):
+    A = T((2, 3), ["a", "b"], rng)
+    B = T((4,), ["c"], rng)
+    P = o.tensor_product(A, B)
+    assert P.labels == ["a", "b", "c"]
+    np.testing.assert_allclose(P.data, np.einsum("ab,c->abc", A.data, B.data))
+    C = T((3, 4, 3), ["p", "q", "r"], rng)
+    C.trace("p", "r")
+    assert C.labels == ["q"]
+
+
+# ---------------------------------------------------------------- con
+def test_con_readme_examples(rng):
+    # tncon.py:26-65 / README.md:81-127
+    A = T((3, 2, 4), ["a", "b", "c"], rng)
+    B = T((3, 4), ["d", "e"], rng)
+    C = T((5, 5, 2), ["f", "g", "h"], rng)
+    r = o.con(A, B, ("a", "d"), ("c", "e"))
+    assert r.labels == ["b"]
+    np.testing.assert_allclose(r.data, np.einsum("abc,ac->b", A.data, B.data), rtol=1e-13)
+    r = o.con(C, ("f", "g"))
+    assert r.labels == ["h"]
+    np.testing.assert_allclose(r.data, np.einsum("ffh->h", C.data), rtol=1e-13)
+    r = o.con(A, B)
+    assert r.shape == (3, 2, 4, 3, 4) and r.labels == ["a", "b", "c", "d", "e"]
+    r = o.con(A, B, C, ("a", "d"), ("c", "e"), ("f", "g"), ("h", "b"))
+    ref = np.einsum("abc,ac,ffb->", A.data, B.data, C.data)
+    np.testing.assert_allclose(r.data, ref, rtol=1e-12)
+
+
+def test_con_ring_closed_form(rng):
+    N = 40
+    a = rng.standard_normal((2, 2)) / 1.5
+    A = o.OTensor(a, ["left", "right"])
+    ts = [A.suf(str(i)) for i in range(N)]
+    pairs = [("right" + str(j), "left" + str(j + 1)) for j in range(N - 1)]
+    r = o.con(ts, pairs, ("right" + str(N - 1), "left0"))
+    np.testing.assert_allclose(r.data, np.trace(np.linalg.matrix_power(a, N)), rtol=1e-10)
+
+
+def test_con_rejects_duplicates(rng):
+    A = T((2, 2), ["a", "b"], rng)
+    B = T((2, 2), ["a", "c"], rng)
+    with pytest.raises(ValueError):
+        o.con(A, B, ("b", "c"))
+
+
+# ---------------------------------------------------------------- decompositions
+def test_tensor_svd_docstring_and_reconstruction(rng):
+    a = T((2, 3, 4), ["i0", "i1", "i2"], rng)
+    U, S, V = o.tensor_svd(a, ["i0", "i2"])
+    assert U.shape == (2, 4, 3) and U.labels == ["i0", "i2", "svd_in"]
+    assert V.shape == (3, 3) and V.labels == ["svd_out", "i1"]
+    assert S.shape == (3, 3) and S.labels == ["svd_out", "svd_in"]
+    b = o.contract(U, o.contract(S, V, "svd_in", "svd_out"), "svd_in", "svd_out")
+    assert o.distance(a, b) < 1e-13
+    mat = np.transpose(a.data, (0, 2, 1)).reshape(8, 3)
+    np.testing.assert_allclose(np.diag(S.data), np.linalg.svd(mat, compute_uv=False), rtol=1e-13)
+
+
+def test_tensor_qr_docstring_and_reconstruction(rng):
+    t = T((2, 3, 4), [], rng)
+    Q, R = o.tensor_qr(t, ["i0", "i2"])
+    assert Q.shape == (2, 4, 3) and Q.labels == ["i0", "i2", "qr_in"]
+    assert R.shape == (3, 3) and R.labels == ["qr_out", "i1"]
+    x = o.contract(Q, R, "qr_in", "qr_out")
+    assert x.labels == ["i0", "i2", "i1"] and o.distance(x, t) < 1e-13
+    q = Q.data.reshape(8, 3)
+    np.testing.assert_allclose(q.T @ q, np.eye(3), atol=1e-14)
+    assert np.allclose(np.tril(R.data, -1), 0)
+
+
+def test_truncated_svd_selection_rule(rng):
+    # closed form: cap at chi, then threshold; truncated_evals = [thr-cut, chi-cut]
+    u, _ = np.linalg.qr(rng.standard_normal((12, 6)))
+    v, _ = np.linalg.qr(rng.standard_normal((10, 6)))
+    s = np.array([5.0, 2.0, 1.0, 1e-3, 1e-9, 1e-12])
+    t = o.OTensor((u * s) @ v.T, ["r", "c"])
+    U, V, ev = o.truncated_svd(t, ["r"], chi=5, threshold=1e-6)
+    assert U.shape == (12, 4) and V.shape == (4, 10)
+    assert len(ev) == 6  # 1 thresholded out of the first 5, then sv[5:] (5 values)
+    np.testing.assert_allclose(ev[:2], [1e-9, 1e-12], rtol=1e-3)
+    assert np.all(ev[2:] < 1e-14)
+    U, V, ev = o.truncated_svd(t, ["r"], chi=0, threshold=1e-4, absolute=False)
+    assert U.shape[1] == 4  # sigma/sigma0 > 1e-4 keeps 5,2,1,1e-3
+    U, S, V = o.truncated_svd(t, ["r"], chi=2, absorb_singular_values=None)
+    np.testing.assert_allclose(np.diag(S.data), [5.0, 2.0], rtol=1e-12)
+
+
+# ---------------------------------------------------------------- one-dimensional
+def random_omps(N, d, chi, rng):
+    arrs = [rng.standard_normal((d, 1 if i == 0 else chi, 1 if i == N - 1 else chi)) for i in range(N)]
+    return o.OMPS([o.OTensor(a, ["phys", "left", "right"]) for a in arrs])
+
+
+@pytest.mark.parametrize("chi", [2, 4, 8])
+def test_svd_compress_equals_tt_rounding_of_dense_state(chi, rng):
+    N, d = 9, 2
+    m = random_omps(N, d, 6, rng)
+    v = om.mps_to_dense(m)
+    m.svd_compress(chi=chi, threshold=1e-15)
+    ref = om.tt_truncate_dense(v, N, d, chi)
+    got = om.mps_to_dense(m)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-11
+    # result is right canonical: all sites but site 0 are right isometries
+    l, r = m.check_canonical_form(threshold=1e-10)
+    assert r == 0
+    assert max(m.bonddims()) <= chi
+
+
+def test_svd_compress_output_axis_order(rng):
+    m = random_omps(5, 2, 3, rng)
+    m.svd_compress(chi=2)
+    assert m[0].labels == ["right", "phys", "left"]
+    assert all(t.labels == ["phys", "right", "left"] for t in m.data[1:])
+
+
+def test_left_canonise_isometry_and_norm(rng):
+    m = random_omps(8, 3, 5, rng)
+    n0 = np.sqrt(o.inner_product_mps(m, m))
+    a = m.copy(); a.left_canonise(qr_decomposition=True)
+    assert a.check_canonical_form(threshold=1e-10)[0] == len(a) - 1
+    np.testing.assert_allclose(a.norm("left"), n0, rtol=1e-12)
+    b = m.copy(); b.left_canonise()
+    np.testing.assert_allclose(b.norm("left"), n0, rtol=1e-12)
+    c = m.copy(); c.left_canonise(normalise=True)
+    np.testing.assert_allclose(c.norm("left"), 1.0, rtol=1e-12)
+    np.testing.assert_allclose(om.mps_to_dense(a), om.mps_to_dense(m), atol=1e-12 * n0)
+
+
+def test_zero_state_special_case(rng):
+    m = random_omps(4, 2, 3, rng)
+    for t in m:
+        t.data[:] = 0
+    m.left_canonise()
+    assert all(t.shape == (2, 1, 1) for t in m)
+    assert all(np.all(t.data == 0) for t in m)
+
+
+def test_contract_mps_mpo_matches_dense(rng):
+    N = 7
+    mpo = om.xxz_mpo(N, Jz=1.0, Jxy=1.0, hz=0.5)
+    H = om.xxz_dense_hamiltonian(N, 1.0, 1.0, 0.5)
+    np.testing.assert_allclose(om.mpo_to_dense(mpo), H, atol=1e-14)
+    m = random_omps(N, 2, 4, rng)
+    out = o.contract_mps_mpo(m, mpo)
+    assert out.phys_label == "physout"
+    assert out[1].labels == ["left", "physout", "right"] and out[1].shape == (20, 2, 20)
+    np.testing.assert_allclose(om.mps_to_dense(out), H @ om.mps_to_dense(m), atol=1e-12)
+
+
+def test_xxz_polarised_energy_closed_form():
+    N, Jz, Jxy, hz = 8, 1.0, 1.0, 0.5
+    up = np.zeros(2 ** N); up[0] = 1.0
+    H = om.xxz_dense_hamiltonian(N, Jz, Jxy, hz)
+    assert abs(up @ H @ up - ((N - 1) * Jz / 4 + N * hz / 2)) < 1e-13
+
+
+def test_inner_product_and_distance_dense(rng):
+    a = random_omps(6, 2, 3, rng)
+    b = random_omps(6, 2, 3, rng)
+    va, vb = om.mps_to_dense(a), om.mps_to_dense(b)
+    np.testing.assert_allclose(o.inner_product_mps(a, b), va @ vb, rtol=1e-12)
+    np.testing.assert_allclose(o.frob_distance_squared(a, b), np.sum((va - vb) ** 2), rtol=1e-11)
+
+
+def test_expvals_dense(rng):
+    N = 6
+    m = random_omps(N, 2, 3, rng)
+    v = om.mps_to_dense(m)
+    v = v / np.linalg.norm(v)
+    m2 = o.OMPS([t * (1.0 / np.linalg.norm(om.mps_to_dense(m)) ** (1.0 / N)) for t in m])
+    op = o.OTensor(om.SZ, ["out", "in"])
+    ev = o.expvals_mps(m2, op)
+    ref = [v @ om._site_op(om.SZ, i, N) @ v for i in range(N)]
+    np.testing.assert_allclose(ev.real, ref, atol=1e-12)
+
+
+# ---------------------------------------------------------------- reference fixture
+def fixture_omps(arrays, meta):
+    return o.OMPS([o.OTensor(a, lab) for a, lab in zip(arrays, meta["labels"])],
+                  meta["left_label"], meta["right_label"], meta["phys_label"])
+
+
+def test_reference_fixture_invariants(fixture_mps_arrays):
+    # tncontract/tests/test_canonical_form_mps_conversion.py:31-64 invariants
+    arrays, meta = fixture_mps_arrays
+    psi = fixture_omps(arrays, meta)
+    n2 = o.inner_product_mps(psi, psi)
+    np.testing.assert_allclose(n2, 0.4668553069796, atol=1e-12)
+    psi.svd_compress(threshold=1e-12, normalise=False)
+    np.testing.assert_almost_equal(psi.norm(), np.sqrt(n2), decimal=10)
+    np.testing.assert_almost_equal(psi.norm("right"), np.sqrt(n2), decimal=10)
+    assert psi.check_canonical_form(threshold=1e-10)[1] == 0
+    psi.left_canonise()
+    assert psi.check_canonical_form(threshold=1e-10)[0] == len(psi) - 1
+    np.testing.assert_almost_equal(psi.norm(), np.sqrt(n2), decimal=10)
+    psi.svd_compress(threshold=1e-12, normalise=True)
+    np.testing.assert_almost_equal(psi.norm(), 1.0, decimal=10)
+    psi2 = fixture_omps(arrays, meta)
+    np.testing.assert_almost_equal(o.inner_product_mps(psi, psi2) ** 2 / n2, 1.0, decimal=10)
