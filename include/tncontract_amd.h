@@ -1,0 +1,120 @@
+/*
+ * tncontract_amd -- C ABI of the MI355X (gfx950) contraction-plus-truncation
+ * library.  Every entry point takes raw device pointers, plain integer shapes
+ * and strides, and a hipStream_t passed as void*.  Buffers are owned by the
+ * caller (the Python layer allocates them with PyTorch-ROCm); work is
+ * stream-ordered; no C++ exception crosses this boundary.  Each function
+ * returns TN_OK (0) or a negative status; tn_last_error() gives the message.
+ *
+ * The reference (andrewdarmawan/tncontract) is pure Python over NumPy/LAPACK
+ * and has no FFI of its own.  Each entry point below names the reference call
+ * site whose arithmetic it replaces; the Python layer (tncontract_amd) binds
+ * them with ctypes behind the reference's own Python API.
+ */
+#ifndef TNCONTRACT_AMD_H
+#define TNCONTRACT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TN_OK 0
+#define TN_ERR_INVALID -1     /* bad shapes / arguments        -> ValueError */
+#define TN_ERR_HIP -2         /* HIP runtime failure           -> RuntimeError */
+#define TN_ERR_NOCONV -3      /* factorisation did not converge -> LinAlgError */
+#define TN_ERR_UNSUPPORTED -4 /* dtype / size not supported    -> NotImplementedError */
+#define TN_ERR_WORKSPACE -5   /* workspace too small           -> ValueError */
+
+#define TN_MAX_MODES 8
+
+/* Element types. */
+#define TN_F64 0
+#define TN_F32 1
+
+/* Arithmetic of a contraction (tn_contract). */
+#define TN_PREC_NATIVE 0 /* same as the element type (FP64 MFMA for TN_F64) */
+#define TN_PREC_F32 1    /* FP32 MFMA, FP32 accumulate                       */
+#define TN_PREC_BF16X3 2 /* split FP32 = hi+lo bf16, 3 bf16 MFMAs, FP32 acc  */
+
+/* A group of tensor modes: element offset of the linear (row-major, slowest
+ * mode first) index i is sum_j idx_j(i) * strides[j].  Strides are in
+ * elements and may be any non-negative value (a permuted view). */
+typedef struct tn_modes {
+  int32_t nmodes;
+  int64_t dims[TN_MAX_MODES];
+  int64_t strides[TN_MAX_MODES];
+} tn_modes;
+
+int tn_version(void);
+const char* tn_last_error(void);
+int tn_last_error_code(void);
+/* Number of HIP devices visible to the runtime this library is bound to. */
+int tn_device_count(int* count);
+
+/* ---- data movement (tensor.py:354-394, 482: np.rollaxis + np.reshape) ----
+ * dst (dense, row-major over src->dims) = src gathered through src->strides. */
+int tn_permute(const void* src, const tn_modes* src_modes, void* dst, int dtype, void* stream);
+
+/* dst = (dst_dtype) src, n elements. */
+int tn_convert(const void* src, int src_dtype, void* dst, int dst_dtype, int64_t n, void* stream);
+
+/* x *= alpha (tensor.py:95-123 scalar multiply; onedim_core.py:358, 482). */
+int tn_scale(void* x, int dtype, int64_t n, double alpha, void* stream);
+
+/* out = a*x + b*y elementwise (Tensor.__add__, tensor.py:125-140; distance,
+ * tensor.py:782-802). */
+int tn_axpby(const void* x, const void* y, void* out, int dtype, int64_t n, double a, double b,
+             void* stream);
+
+/* out (k x k, row-major) = diag(s) (S = np.diag(s) in tensor_svd, tensor.py:940). */
+int tn_diag_embed(const double* s, int64_t k, void* out, int dtype, void* stream);
+
+/* *out (device double) = sum x_i^2 in FP64 (np.linalg.norm, tensor.py:590;
+ * onedim_core.py:272, 305, 656).  ws: >= tn_sumsq_workspace(n) bytes. */
+size_t tn_sumsq_workspace(int64_t n);
+int tn_sumsq(const void* x, int dtype, int64_t n, double* out, void* ws, size_t ws_bytes,
+             void* stream);
+
+/* Internal contraction (np.trace, tensor.py:318-338):
+ * dst[o] = sum_{t<diag_dim} src[off(out_modes, o) + t * diag_stride]. */
+int tn_trace(const void* src, const tn_modes* out_modes, int64_t diag_dim, int64_t diag_stride,
+             void* dst, int dtype, void* stream);
+
+/* ---- pairwise contraction (np.tensordot in contract(), tensor.py:735) ----
+ * C[m,n] = alpha * sum_k A[m,k] B[k,n] + beta * C[m,n], where the row/col
+ * index of each operand is decoded through its mode group, so any label
+ * permutation of the inputs and of the output is fused into the loads and the
+ * stores.  a_k and b_k must have the same total size.  precision: TN_PREC_*.
+ * beta == 0 never reads C. */
+int tn_contract(const void* A, const tn_modes* a_m, const tn_modes* a_k,
+                const void* B, const tn_modes* b_k, const tn_modes* b_n,
+                void* C, const tn_modes* c_m, const tn_modes* c_n,
+                int dtype, int precision, double alpha, double beta, void* stream);
+
+/* ---- QR (np.linalg.qr(mode="reduced") in tensor_qr, tensor.py:1044) ------
+ * A is the m x n matrix whose row index is decoded through a_rows and column
+ * index through a_cols.  Q: m x k row-major, R: k x n row-major, k=min(m,n).
+ * Blocked Householder with FP64 reflectors; R's diagonal follows the LAPACK
+ * dgeqrf sign convention (beta = -sign(alpha) * norm).  dtype TN_F64. */
+size_t tn_qr_workspace(int64_t m, int64_t n, int dtype);
+int tn_qr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m, int64_t n,
+          void* Q, void* R, int dtype, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- SVD (np.linalg.svd(full_matrices=False), tensor.py:915) ------------
+ * Thin SVD of the m x n matrix A (decoded as in tn_qr): U m x k row-major,
+ * S k (FP64, descending), Vt k x n row-major, k = min(m,n).  One-sided
+ * block Jacobi with FP64 rotations.  tol: orthogonality tolerance (<=0 ->
+ * 1e-14); max_sweeps <= 0 -> 40.  *sweeps_out (host, may be NULL) receives
+ * the number of sweeps used.  Returns TN_ERR_NOCONV if not converged. */
+size_t tn_svd_workspace(int64_t m, int64_t n, int dtype);
+int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m, int64_t n,
+           void* U, double* S, void* Vt, int dtype, double tol, int max_sweeps, int* sweeps_out,
+           void* ws, size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNCONTRACT_AMD_H */

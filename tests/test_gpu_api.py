@@ -1,0 +1,222 @@
+"""Parity of the drop-in API (tncontract_amd) with the CPU oracle, on the GPU.
+
+Bar: FP64 results equal to the oracle to <= 1e-10 (relative), and every MPS
+result compared through gauge-invariant quantities: the state error
+||psi_gpu - psi_oracle||^2 / ||psi_oracle||^2, norms, overlaps, singular
+values and <Sz_i>."""
+import numpy as np
+import pytest
+
+from oracle import tn_oracle as o
+from oracle import models as om
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def tn():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import tncontract_amd
+    tncontract_amd.native_library()
+    return tncontract_amd
+
+
+def both(tn, arr, labels):
+    return tn.Tensor(arr, labels), o.OTensor(arr, labels)
+
+
+def test_contract_and_shorthand(tn):
+    rng = np.random.default_rng(0)
+    A, oA = both(tn, rng.standard_normal((2, 2)), ["spam", "eggs"])
+    B, oB = both(tn, rng.standard_normal((2, 3, 2, 4)), [])
+    C = tn.contract(A, B, "spam", "i2")
+    oC = o.contract(oA, oB, "spam", "i2")
+    assert C.labels == oC.labels == ["eggs", "i0", "i1", "i3"]
+    np.testing.assert_allclose(C.numpy(), oC.data, rtol=1e-13)
+    D = A["spam", "eggs"] * B["i0", "i2"]
+    oD = o.contract(oA, oB, ["spam", "eggs"], ["i0", "i2"])
+    assert D.labels == oD.labels
+    np.testing.assert_allclose(D.numpy(), oD.data, rtol=1e-13)
+
+
+def test_contract_duplicate_labels_slices_and_errors(tn):
+    rng = np.random.default_rng(1)
+    A, oA = both(tn, rng.standard_normal((3, 4, 3)), ["x", "y", "x"])
+    B, oB = both(tn, rng.standard_normal((3, 5, 3)), ["x", "z", "x"])
+    np.testing.assert_allclose(tn.contract(A, B, "x", "x").numpy(), o.contract(oA, oB, "x", "x").data,
+                               rtol=1e-13)
+    C = tn.contract(A, B, "x", "x", index_slice1=[-1], index_slice2=[0])
+    oC = o.contract(oA, oB, "x", "x", index_slice1=[-1], index_slice2=[0])
+    assert C.labels == oC.labels
+    np.testing.assert_allclose(C.numpy(), oC.data, rtol=1e-13)
+    with pytest.raises(ValueError):
+        tn.contract(A, B, "y", "z")
+
+
+def test_tensordot_numpy_semantics(tn):
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((4, 5, 6))
+    b = rng.standard_normal((6, 5, 3))
+    np.testing.assert_allclose(tn.tensordot(a, b, ([1, 2], [1, 0])).cpu().numpy(),
+                               np.tensordot(a, b, ([1, 2], [1, 0])), rtol=1e-13)
+    np.testing.assert_allclose(tn.tensordot(a, b[:, :, :1].reshape(6, 5), ([2, 1], [0, 1])).cpu().numpy(),
+                               np.tensordot(a, b[:, :, :1].reshape(6, 5), ([2, 1], [0, 1])), rtol=1e-13)
+
+
+def test_axis_ops_and_consolidate(tn):
+    rng = np.random.default_rng(3)
+    arr = rng.standard_normal((2, 3, 4, 5))
+    t, ot = both(tn, arr, ["z", "a", "z", "b"])
+    t.consolidate_indices()
+    ot.consolidate_indices()
+    assert t.labels == ot.labels and t.shape == ot.shape
+    np.testing.assert_array_equal(t.numpy(), ot.data)
+    t, ot = both(tn, rng.standard_normal((2, 3, 4, 5, 6)), ["a", "b", "c", "b", "d"])
+    t.move_indices(["d", "b", "c"], 0)
+    ot.move_indices(["d", "b", "c"], 0)
+    assert t.labels == ot.labels
+    np.testing.assert_array_equal(t.numpy(), ot.data)
+
+
+def test_trace_and_con(tn):
+    rng = np.random.default_rng(4)
+    A, oA = both(tn, rng.standard_normal((3, 2, 4)), ["a", "b", "c"])
+    B, oB = both(tn, rng.standard_normal((3, 4)), ["d", "e"])
+    C, oC = both(tn, rng.standard_normal((5, 5, 2)), ["f", "g", "h"])
+    r = tn.con(A, B, C, ("a", "d"), ("c", "e"), ("f", "g"), ("h", "b"))
+    orr = o.con(oA, oB, oC, ("a", "d"), ("c", "e"), ("f", "g"), ("h", "b"))
+    np.testing.assert_allclose(r.numpy(), orr.data, rtol=1e-12)
+    N = 30
+    a = rng.standard_normal((2, 2)) / 1.5
+    ts = [tn.Tensor(a, ["left", "right"]).suf(str(i)) for i in range(N)]
+    pairs = [("right" + str(j), "left" + str(j + 1)) for j in range(N - 1)]
+    r = tn.con(ts, pairs, ("right" + str(N - 1), "left0"))
+    np.testing.assert_allclose(r.numpy(), np.trace(np.linalg.matrix_power(a, N)), rtol=1e-10)
+
+
+def test_tensor_svd_qr_truncated(tn):
+    rng = np.random.default_rng(5)
+    arr = rng.standard_normal((6, 7, 8))
+    t, ot = both(tn, arr, ["i0", "i1", "i2"])
+    U, S, V = tn.tensor_svd(t, ["i0", "i2"])
+    oU, oS, oV = o.tensor_svd(ot, ["i0", "i2"])
+    assert U.labels == oU.labels and U.shape == oU.shape and V.labels == oV.labels
+    np.testing.assert_allclose(np.diag(S.numpy()), np.diag(oS.data), rtol=1e-12)
+    rec = tn.contract(U, tn.contract(S, V, "svd_in", "svd_out"), "svd_in", "svd_out")
+    assert tn.distance(rec, t) < 1e-12 * np.linalg.norm(arr)
+    Q, R = tn.tensor_qr(t, ["i0", "i2"])
+    oQ, oR = o.tensor_qr(ot, ["i0", "i2"])
+    assert Q.labels == oQ.labels and R.labels == oR.labels and Q.shape == oQ.shape
+    np.testing.assert_allclose(R.numpy(), oR.data, atol=1e-11)
+    u, _ = np.linalg.qr(rng.standard_normal((12, 6)))
+    v, _ = np.linalg.qr(rng.standard_normal((10, 6)))
+    s = np.array([5.0, 2.0, 1.0, 1e-3, 1e-9, 1e-12])
+    m = (u * s) @ v.T
+    for kw in [dict(chi=5, threshold=1e-6), dict(chi=0, threshold=1e-4, absolute=False),
+               dict(chi=3, absorb_singular_values="left"), dict(chi=4, absorb_singular_values="both")]:
+        g = tn.truncated_svd(tn.Tensor(m, ["r", "c"]), ["r"], **kw)
+        r = o.truncated_svd(o.OTensor(m, ["r", "c"]), ["r"], **kw)
+        assert g[0].shape == r[0].shape and g[1].shape == r[1].shape
+        np.testing.assert_allclose(np.sort(np.abs(g[2])), np.sort(np.abs(r[2])), rtol=1e-6, atol=1e-14)
+        prod_g = tn.contract(g[0], g[1], "svd_in", "svd_out").numpy()
+        prod_o = o.contract(r[0], r[1], "svd_in", "svd_out").data
+        np.testing.assert_allclose(prod_g, prod_o, atol=1e-12)
+
+
+def gpu_mps(tn, arrs, labels=("phys", "left", "right")):
+    return tn.onedim.MatrixProductState([tn.Tensor(a, list(labels)) for a in arrs])
+
+
+def o_mps(arrs, labels=("phys", "left", "right")):
+    return o.OMPS([o.OTensor(a, list(labels)) for a in arrs])
+
+
+def to_dense_gpu(mps):
+    return om.mps_to_dense(o.OMPS([o.OTensor(t.numpy(), t.labels) for t in mps],
+                                  mps.left_label, mps.right_label, mps.phys_label))
+
+
+def state_err(a, b):
+    return np.linalg.norm(a - b) ** 2 / np.linalg.norm(b) ** 2
+
+
+@pytest.mark.parametrize("chi", [2, 5, 16])
+def test_svd_compress_matches_oracle(tn, chi):
+    from tncontract_amd.onedim import random_mps_arrays
+    arrs = random_mps_arrays(10, 2, 8, seed=chi)
+    g, r = gpu_mps(tn, arrs), o_mps(arrs)
+    g.svd_compress(chi=chi, threshold=1e-10)
+    r.svd_compress(chi=chi, threshold=1e-10)
+    assert [t.labels for t in g] == [t.labels for t in r]
+    assert g.bonddims() == r.bonddims()
+    assert state_err(to_dense_gpu(g), om.mps_to_dense(r)) < 1e-10
+    assert g.check_canonical_form(threshold=1e-10, print_output=False)[1] == 0
+
+
+def test_left_canonise_svd_and_qr(tn):
+    from tncontract_amd.onedim import random_mps_arrays
+    arrs = random_mps_arrays(8, 3, 6, seed=7)
+    for kw in [dict(qr_decomposition=True), dict(), dict(normalise=True), dict(chi=3, threshold=1e-8)]:
+        g, r = gpu_mps(tn, arrs), o_mps(arrs)
+        g.left_canonise(**kw)
+        r.left_canonise(**kw)
+        assert [t.labels for t in g] == [t.labels for t in r]
+        assert g.bonddims() == r.bonddims()
+        assert state_err(to_dense_gpu(g), om.mps_to_dense(r)) < 1e-10
+        np.testing.assert_allclose(g.norm(), r.norm(), rtol=1e-10)
+
+
+def test_contract_mps_mpo_and_compress_config_small(tn):
+    from tncontract_amd.onedim import random_mps_arrays, xxz_mpo
+    N = 8
+    arrs = random_mps_arrays(N, 2, 6, seed=1)
+    g, r = gpu_mps(tn, arrs), o_mps(arrs)
+    gm = tn.onedim.contract_mps_mpo(g, xxz_mpo(N))
+    rm = o.contract_mps_mpo(r, om.xxz_mpo(N))
+    assert [t.labels for t in gm] == [t.labels for t in rm]
+    assert [t.shape for t in gm] == [t.shape for t in rm]
+    for a, b in zip(gm, rm):
+        np.testing.assert_allclose(a.numpy(), b.data, rtol=1e-13, atol=1e-15)
+    gm.svd_compress(chi=6)
+    rm.svd_compress(chi=6)
+    assert state_err(to_dense_gpu(gm), om.mps_to_dense(rm)) < 1e-10
+
+
+def test_inner_product_distance_expvals(tn):
+    from tncontract_amd.onedim import random_mps_arrays
+    a1 = random_mps_arrays(7, 2, 4, seed=2)
+    a2 = random_mps_arrays(7, 2, 4, seed=3)
+    g1, g2 = gpu_mps(tn, a1), gpu_mps(tn, a2)
+    r1, r2 = o_mps(a1), o_mps(a2)
+    np.testing.assert_allclose(tn.onedim.inner_product_mps(g1, g2), o.inner_product_mps(r1, r2), rtol=1e-11)
+    np.testing.assert_allclose(tn.onedim.frob_distance_squared(g1, g2), o.frob_distance_squared(r1, r2),
+                               rtol=1e-10)
+    op = tn.Tensor(om.SZ, ["out", "in"])
+    ev = tn.onedim.expvals_mps(g1, op)
+    oev = o.expvals_mps(r1, o.OTensor(om.SZ, ["out", "in"]))
+    np.testing.assert_allclose(ev.real, oev.real, atol=1e-11)
+
+
+def test_reference_fixture_invariants_gpu(tn, fixture_mps_arrays):
+    # tncontract/tests/test_canonical_form_mps_conversion.py invariants
+    arrays, meta = fixture_mps_arrays
+    psi = tn.onedim.MatrixProductState([tn.Tensor(a, lab) for a, lab in zip(arrays, meta["labels"])],
+                                       meta["left_label"], meta["right_label"], meta["phys_label"])
+    n2 = float(tn.onedim.inner_product_mps(psi, psi))
+    np.testing.assert_allclose(n2, 0.4668553069796, atol=1e-12)
+    psi.svd_compress(threshold=1e-12, normalise=False)
+    np.testing.assert_almost_equal(psi.norm(), np.sqrt(n2), decimal=10)
+    assert psi.check_canonical_form(threshold=1e-10, print_output=False)[1] == 0
+    psi.left_canonise()
+    assert psi.check_canonical_form(threshold=1e-10, print_output=False)[0] == len(psi) - 1
+    psi.svd_compress(threshold=1e-12, normalise=True)
+    np.testing.assert_almost_equal(psi.norm(), 1.0, decimal=10)
+
+
+def test_zero_state(tn):
+    arrs = [np.zeros((2, 1 if i == 0 else 3, 1 if i == 3 else 3)) for i in range(4)]
+    g = gpu_mps(tn, arrs)
+    g.left_canonise()
+    assert all(t.shape == (2, 1, 1) for t in g)

@@ -1,0 +1,599 @@
+// Pairwise tensor contraction (np.tensordot in contract(), tensor.py:735).
+//
+// Two kernels:
+//  * gemm_mfma<Prec>: C[m,n] = alpha*sum_k A[m,k]B[k,n] + beta*C, 128x128
+//    block tile, 4 waves (2x2) each owning a 64x64 tile of 16x16 MFMA
+//    fragments.  The row/col/k index of every operand is decoded through its
+//    mode group, so a label permutation of A, B or C is fused into the
+//    global->LDS tile loads and into the epilogue stores (no transpose copy).
+//    Prec = FP64 (v_mfma_f64_16x16x4), FP32 (v_mfma_f32_16x16x4) or BF16X3
+//    (each FP32 operand split into hi+lo bf16, three v_mfma_f32_16x16x32_bf16
+//    per tile product: hi*hi + hi*lo + lo*hi, FP32 accumulation).
+//  * contract_direct: one thread per output element, iterated in the
+//    output's memory order; used when K is small (e.g. the K = d = 2 MPO
+//    application), where the work is HBM-bound on the stores.
+#include "tn_internal.h"
+
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
+namespace tn {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+enum { P_F64 = 0, P_F32 = 1, P_BF16X3 = 2 };
+
+constexpr int BM = 128, BN = 128;
+constexpr int NT = 256;
+
+struct GemmArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  Modes am, ak, bk, bn, cm, cn;
+  int64_t M, N, K;
+  double alpha, beta;
+  int a_kfast;  // A tile loads iterate k fastest (A unit stride along k)
+  int b_nfast;  // B tile loads iterate n fastest
+  int64_t kchunk;  // split-K: block z handles k in [z*kchunk, (z+1)*kchunk)
+  void* part;      // split-K: dense M x N partial sums per z (nullptr: write C)
+};
+
+// Offset of index i in a mode group; single-mode groups avoid division.
+__device__ __forceinline__ int64_t goff(const Modes& m, int64_t i) {
+  if (m.n == 1) return i * m.strides[0];
+  if (m.n == 0) return 0;
+  return mode_offset(m, i);
+}
+
+template <int P> struct PrecTraits;
+template <> struct PrecTraits<P_F64> { using T = double; static constexpr int BK = 16; };
+template <> struct PrecTraits<P_F32> { using T = float; static constexpr int BK = 16; };
+template <> struct PrecTraits<P_BF16X3> { using T = float; static constexpr int BK = 32; };
+
+// ------------------------------------------------------------ FP64 / FP32
+// LDS layout [BK][BM + 2]: fragment reads by 16 lanes hit 16 consecutive rows.
+template <int P>
+__global__ void __launch_bounds__(NT) gemm_mfma_kernel(GemmArgs g) {
+  using T = typename PrecTraits<P>::T;
+  constexpr int BK = PrecTraits<P>::BK;
+  constexpr int LDA = BM + 2, LDB = BN + 2;
+  constexpr int EPT_A = BM * BK / NT, EPT_B = BN * BK / NT;  // 8 each
+  __shared__ T As[BK * LDA];
+  __shared__ T Bs[BK * LDB];
+  __shared__ int64_t rowA[BM], colB[BN], rowC[BM], colC[BN];
+
+  const T* __restrict__ A = (const T*)g.A;
+  const T* __restrict__ B = (const T*)g.B;
+  T* __restrict__ C = (T*)g.C;
+  const int tid = threadIdx.x;
+  const int64_t m0 = (int64_t)blockIdx.y * BM, n0 = (int64_t)blockIdx.x * BN;
+  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
+  const int64_t kend = min(g.K, kbeg + g.kchunk);
+
+  for (int i = tid; i < BM; i += NT) {
+    int64_t m = m0 + i;
+    rowA[i] = m < g.M ? goff(g.am, m) : 0;
+    rowC[i] = m < g.M ? goff(g.cm, m) : 0;
+  }
+  for (int i = tid; i < BN; i += NT) {
+    int64_t n = n0 + i;
+    colB[i] = n < g.N ? goff(g.bn, n) : 0;
+    colC[i] = n < g.N ? goff(g.cn, n) : 0;
+  }
+  __syncthreads();
+
+  T ra[EPT_A], rb[EPT_B];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int j = 0; j < EPT_A; ++j) {
+      int e = tid + NT * j;
+      int r, kk;
+      if (g.a_kfast) { kk = e % BK; r = e / BK; } else { r = e % BM; kk = e / BM; }
+      int64_t m = m0 + r, k = k0 + kk;
+      ra[j] = (m < g.M && k < kend) ? A[rowA[r] + goff(g.ak, k)] : (T)0;
+    }
+#pragma unroll
+    for (int j = 0; j < EPT_B; ++j) {
+      int e = tid + NT * j;
+      int c, kk;
+      if (g.b_nfast) { c = e % BN; kk = e / BN; } else { kk = e % BK; c = e / BK; }
+      int64_t n = n0 + c, k = k0 + kk;
+      rb[j] = (n < g.N && k < kend) ? B[goff(g.bk, k) + colB[c]] : (T)0;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int j = 0; j < EPT_A; ++j) {
+      int e = tid + NT * j;
+      int r, kk;
+      if (g.a_kfast) { kk = e % BK; r = e / BK; } else { r = e % BM; kk = e / BM; }
+      As[kk * LDA + r] = ra[j];
+    }
+#pragma unroll
+    for (int j = 0; j < EPT_B; ++j) {
+      int e = tid + NT * j;
+      int c, kk;
+      if (g.b_nfast) { c = e % BN; kk = e / BN; } else { kk = e % BK; c = e / BK; }
+      Bs[kk * LDB + c] = rb[j];
+    }
+  };
+
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  using Acc = typename std::conditional<P == P_F64, f64x4, f32x4>::type;
+  Acc acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+
+  const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  if (nk > 0) {
+    load(kbeg);
+    store();
+    __syncthreads();
+  }
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      T af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[(ks + lk) * LDA + wr + 16 * i + lr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Bs[(ks + lk) * LDB + wc + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if constexpr (P == P_F64)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store();
+      __syncthreads();
+    }
+  }
+
+  // epilogue: FP64 fragment element e of lane l sits at row (l/16)+4e,
+  // FP32 at row 4*(l/16)+e; column l%16 in both.
+  const T alpha = (T)g.alpha, beta = (T)g.beta;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int r = wr + 16 * i + (P == P_F64 ? (lk + 4 * e) : (4 * lk + e));
+        int c = wc + 16 * j + lr;
+        int64_t m = m0 + r, n = n0 + c;
+        if (m < g.M && n < g.N) {
+          if (g.part) {
+            ((T*)g.part)[((int64_t)blockIdx.z * g.M + m) * g.N + n] = acc[i][j][e];
+          } else {
+            T* p = C + rowC[r] + colC[c];
+            T v = alpha * acc[i][j][e];
+            if (g.beta != 0.0) v += beta * *p;
+            *p = v;
+          }
+        }
+      }
+}
+
+// ------------------------------------------------------------ BF16x3
+// LDS layout [row][BK + 8] bf16 (k contiguous): lane l of a 16x16x32 MFMA
+// reads 8 consecutive k of row l%16 starting at k = 8*(l/16).
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+__global__ void __launch_bounds__(NT) gemm_bf16x3_kernel(GemmArgs g) {
+  constexpr int BK = 32;
+  constexpr int LD = BK + 8;
+  constexpr int EPT = BM * BK / NT;  // 16
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM * LD], Al[BM * LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * LD], Bl[BN * LD];
+  __shared__ int64_t rowA[BM], colB[BN], rowC[BM], colC[BN];
+
+  const float* __restrict__ A = (const float*)g.A;
+  const float* __restrict__ B = (const float*)g.B;
+  float* __restrict__ C = (float*)g.C;
+  const int tid = threadIdx.x;
+  const int64_t m0 = (int64_t)blockIdx.y * BM, n0 = (int64_t)blockIdx.x * BN;
+  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
+  const int64_t kend = min(g.K, kbeg + g.kchunk);
+  for (int i = tid; i < BM; i += NT) {
+    int64_t m = m0 + i;
+    rowA[i] = m < g.M ? goff(g.am, m) : 0;
+    rowC[i] = m < g.M ? goff(g.cm, m) : 0;
+  }
+  for (int i = tid; i < BN; i += NT) {
+    int64_t n = n0 + i;
+    colB[i] = n < g.N ? goff(g.bn, n) : 0;
+    colC[i] = n < g.N ? goff(g.cn, n) : 0;
+  }
+  __syncthreads();
+
+  float ra[EPT], rb[EPT];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      int e = tid + NT * j;
+      int r, kk;
+      if (g.a_kfast) { kk = e % BK; r = e / BK; } else { r = e % BM; kk = e / BM; }
+      int64_t m = m0 + r, k = k0 + kk;
+      ra[j] = (m < g.M && k < kend) ? A[rowA[r] + goff(g.ak, k)] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      int e = tid + NT * j;
+      int c, kk;
+      if (g.b_nfast) { c = e % BN; kk = e / BN; } else { kk = e % BK; c = e / BK; }
+      int64_t n = n0 + c, k = k0 + kk;
+      rb[j] = (n < g.N && k < kend) ? B[goff(g.bk, k) + colB[c]] : 0.f;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      int e = tid + NT * j;
+      int r, kk;
+      if (g.a_kfast) { kk = e % BK; r = e / BK; } else { r = e % BM; kk = e / BM; }
+      split_bf16(ra[j], Ah[r * LD + kk], Al[r * LD + kk]);
+    }
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      int e = tid + NT * j;
+      int c, kk;
+      if (g.b_nfast) { c = e % BN; kk = e / BN; } else { kk = e % BK; c = e / BK; }
+      split_bf16(rb[j], Bh[c * LD + kk], Bl[c * LD + kk]);
+    }
+  };
+
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+
+  const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  if (nk > 0) {
+    load(kbeg);
+    store();
+    __syncthreads();
+  }
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+    bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ah[i] = *(const bf16x8*)&Ah[(wr + 16 * i + lr) * LD + lk];
+      al[i] = *(const bf16x8*)&Al[(wr + 16 * i + lr) * LD + lk];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bh[j] = *(const bf16x8*)&Bh[(wc + 16 * j + lr) * LD + lk];
+      bl[j] = *(const bf16x8*)&Bl[(wc + 16 * j + lr) * LD + lk];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store();
+      __syncthreads();
+    }
+  }
+  const float alpha = (float)g.alpha, beta = (float)g.beta;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int r = wr + 16 * i + 4 * (lane >> 4) + e;
+        int c = wc + 16 * j + lr;
+        int64_t m = m0 + r, n = n0 + c;
+        if (m < g.M && n < g.N) {
+          if (g.part) {
+            ((float*)g.part)[((int64_t)blockIdx.z * g.M + m) * g.N + n] = acc[i][j][e];
+          } else {
+            float* p = C + rowC[r] + colC[c];
+            float v = alpha * acc[i][j][e];
+            if (g.beta != 0.0) v += beta * *p;
+            *p = v;
+          }
+        }
+      }
+}
+
+// ------------------------------------------------------------ direct
+struct DirectArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  int no, nk;
+  int64_t odim[2 * TN_MAX_MODES], oa[2 * TN_MAX_MODES], ob[2 * TN_MAX_MODES],
+      oc[2 * TN_MAX_MODES];
+  int64_t kdim[TN_MAX_MODES], ka[TN_MAX_MODES], kb[TN_MAX_MODES];
+  int64_t nout, K;
+  double alpha, beta;
+};
+
+template <typename T, typename Acc>
+__global__ void __launch_bounds__(256) contract_direct(DirectArgs g) {
+  const T* __restrict__ A = (const T*)g.A;
+  const T* __restrict__ B = (const T*)g.B;
+  T* __restrict__ C = (T*)g.C;
+  int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; o < g.nout; o += stride) {
+    int64_t idx = o, pa = 0, pb = 0, pc = 0;
+    for (int i = g.no - 1; i >= 0; --i) {
+      int64_t d = g.odim[i], q = idx / d, r = idx - q * d;
+      pa += r * g.oa[i];
+      pb += r * g.ob[i];
+      pc += r * g.oc[i];
+      idx = q;
+    }
+    Acc acc = 0;
+    for (int64_t k = 0; k < g.K; ++k) {
+      int64_t kidx = k, qa = 0, qb = 0;
+      for (int i = g.nk - 1; i >= 0; --i) {
+        int64_t d = g.kdim[i], q = kidx / d, r = kidx - q * d;
+        qa += r * g.ka[i];
+        qb += r * g.kb[i];
+        kidx = q;
+      }
+      acc += (Acc)A[pa + qa] * (Acc)B[pb + qb];
+    }
+    T v = (T)(g.alpha * (double)acc);
+    if (g.beta != 0.0) v += (T)g.beta * C[pc];
+    C[pc] = v;
+  }
+}
+
+// Merge adjacent modes of a group when the merge is valid for every operand
+// that shares the group (strides of the slower mode = stride*dim of the faster).
+static void merge_group(std::vector<int64_t>& dims, std::vector<std::vector<int64_t>*> strides) {
+  size_t i = 0;
+  // drop unit dims first
+  for (size_t j = 0; j < dims.size();) {
+    if (dims[j] == 1 && dims.size() > 1) {
+      dims.erase(dims.begin() + j);
+      for (auto* s : strides) s->erase(s->begin() + j);
+    } else {
+      ++j;
+    }
+  }
+  while (i + 1 < dims.size()) {
+    bool ok = true;
+    for (auto* s : strides)
+      if ((*s)[i] != (*s)[i + 1] * dims[i + 1]) ok = false;
+    if (ok) {
+      dims[i] *= dims[i + 1];
+      for (auto* s : strides) {
+        (*s)[i] = (*s)[i + 1];
+        s->erase(s->begin() + i + 1);
+      }
+      dims.erase(dims.begin() + i + 1);
+    } else {
+      ++i;
+    }
+  }
+}
+
+static Modes make_modes(const std::vector<int64_t>& d, const std::vector<int64_t>& s) {
+  Modes m{};
+  m.n = (int)d.size();
+  for (int i = 0; i < TN_MAX_MODES; ++i) {
+    m.dims[i] = i < m.n ? d[i] : 1;
+    m.strides[i] = i < m.n ? s[i] : 0;
+  }
+  return m;
+}
+
+static void unpack(const tn_modes* m, std::vector<int64_t>& d, std::vector<int64_t>& s) {
+  d.clear();
+  s.clear();
+  if (!m) return;
+  for (int i = 0; i < m->nmodes; ++i) {
+    d.push_back(m->dims[i]);
+    s.push_back(m->strides[i]);
+  }
+}
+
+static int64_t prod(const std::vector<int64_t>& d) {
+  int64_t p = 1;
+  for (auto x : d) p *= x;
+  return p;
+}
+
+static bool dims_equal(const tn_modes* a, const tn_modes* b) {
+  int na = a ? a->nmodes : 0, nb = b ? b->nmodes : 0;
+  if (na != nb) return false;
+  for (int i = 0; i < na; ++i)
+    if (a->dims[i] != b->dims[i]) return false;
+  return true;
+}
+
+
+// Split-K reduction: C = alpha * sum_z part[z] + beta * C.
+template <typename T>
+__global__ void __launch_bounds__(256) splitk_reduce(const T* __restrict__ part, int S, int64_t M,
+                                                     int64_t N, Modes cm, Modes cn, T* C,
+                                                     double alpha, double beta) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t MN = M * N;
+  if (i >= MN) return;
+  double acc = 0.0;
+  for (int z = 0; z < S; ++z) acc += (double)part[(int64_t)z * MN + i];
+  int64_t m = i / N, n = i - m * N;
+  T* p = C + goff(cm, m) + goff(cn, n);
+  double v = alpha * acc;
+  if (beta != 0.0) v += beta * (double)*p;
+  *p = (T)v;
+}
+
+int contract_launch(const void* A, const Modes& a_m, const Modes& a_k, const void* B,
+                    const Modes& b_k, const Modes& b_n, void* C, const Modes& c_m,
+                    const Modes& c_n, int dtype, int prec, double alpha, double beta,
+                    void* splitk_ws, size_t splitk_bytes, hipStream_t st) {
+  if (prec == TN_PREC_NATIVE) prec = dtype == TN_F64 ? P_F64 : P_F32;
+  TN_REQUIRE(dtype == TN_F64 || dtype == TN_F32, "contract: unsupported dtype %d", dtype);
+  TN_REQUIRE(!(dtype == TN_F64 && prec != P_F64),
+             "contract: FP64 data supports only native precision (convert to FP32 first)");
+  TN_REQUIRE(!(dtype == TN_F32 && prec == P_F64), "contract: FP32 data cannot use FP64 MFMA");
+  std::vector<int64_t> md, mas, mcs, nd, nbs, ncs, kd, kas, kbs;
+  for (int i = 0; i < a_m.n; ++i) { md.push_back(a_m.dims[i]); mas.push_back(a_m.strides[i]); mcs.push_back(c_m.strides[i]); }
+  for (int i = 0; i < b_n.n; ++i) { nd.push_back(b_n.dims[i]); nbs.push_back(b_n.strides[i]); ncs.push_back(c_n.strides[i]); }
+  for (int i = 0; i < a_k.n; ++i) { kd.push_back(a_k.dims[i]); kas.push_back(a_k.strides[i]); kbs.push_back(b_k.strides[i]); }
+  const int64_t M = prod(md), N = prod(nd), K = prod(kd);
+  if (M == 0 || N == 0) return TN_OK;
+  TN_REQUIRE(A && B && C, "contract: null pointer");
+
+  bool small_k = K <= 8;
+  bool tiny = (double)M * (double)N * (double)std::max<int64_t>(K, 1) < 64.0 * 64.0 * 64.0;
+  if (K == 0 || small_k || tiny || (M < 16 && N < 16)) {
+    // direct kernel: output modes sorted by C stride (descending) so that
+    // consecutive threads store consecutive addresses.
+    struct OM { int64_t d, a, b, c; };
+    std::vector<OM> om;
+    for (size_t i = 0; i < md.size(); ++i) om.push_back({md[i], mas[i], 0, mcs[i]});
+    for (size_t i = 0; i < nd.size(); ++i) om.push_back({nd[i], 0, nbs[i], ncs[i]});
+    std::stable_sort(om.begin(), om.end(), [](const OM& x, const OM& y) { return x.c > y.c; });
+    std::vector<OM> om2;
+    for (auto& x : om)
+      if (x.d != 1) om2.push_back(x);
+    TN_REQUIRE(om2.size() <= 2 * TN_MAX_MODES, "contract: too many output modes");
+    merge_group(kd, {&kas, &kbs});
+    TN_REQUIRE(kd.size() <= TN_MAX_MODES, "contract: too many contracted modes");
+    DirectArgs g{};
+    g.A = A; g.B = B; g.C = C;
+    g.no = (int)om2.size();
+    for (int i = 0; i < g.no; ++i) {
+      g.odim[i] = om2[i].d; g.oa[i] = om2[i].a; g.ob[i] = om2[i].b; g.oc[i] = om2[i].c;
+    }
+    g.nk = K == 0 ? 0 : (int)kd.size();
+    for (int i = 0; i < g.nk; ++i) { g.kdim[i] = kd[i]; g.ka[i] = kas[i]; g.kb[i] = kbs[i]; }
+    g.nout = M * N;
+    g.K = K;
+    g.alpha = alpha;
+    g.beta = beta;
+    unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(g.nout, 256), 65536 * 4);
+    if (dtype == TN_F64)
+      hipLaunchKernelGGL((contract_direct<double, double>), dim3(blocks), dim3(256), 0, st, g);
+    else
+      hipLaunchKernelGGL((contract_direct<float, float>), dim3(blocks), dim3(256), 0, st, g);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+  }
+
+  merge_group(md, {&mas, &mcs});
+  merge_group(nd, {&nbs, &ncs});
+  merge_group(kd, {&kas, &kbs});
+  GemmArgs g{};
+  g.A = A; g.B = B; g.C = C;
+  g.am = make_modes(md, mas);
+  g.cm = make_modes(md, mcs);
+  g.bn = make_modes(nd, nbs);
+  g.cn = make_modes(nd, ncs);
+  g.ak = make_modes(kd, kas);
+  g.bk = make_modes(kd, kbs);
+  g.M = M; g.N = N; g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  // Lanes of a tile load walk the operand's unit-stride dimension.
+  const bool a_k_unit = !kas.empty() && kas.back() == 1;
+  const bool b_k_unit = !kbs.empty() && kbs.back() == 1;
+  const bool b_n_unit = !nbs.empty() && nbs.back() == 1;
+  g.a_kfast = a_k_unit ? 1 : 0;
+  g.b_nfast = (b_k_unit && !b_n_unit) ? 0 : 1;
+  TN_REQUIRE(ceil_div(M, BM) <= 65535, "contract: M too large");
+  const int64_t tiles = ceil_div(M, BM) * ceil_div(N, BN);
+  // Split K when few output tiles would leave most CUs idle.
+  int S = 1;
+  const size_t esz = dtype == TN_F64 ? 8 : 4;
+  if (splitk_ws != nullptr && tiles < 128 && K >= 1024) {
+    S = (int)std::min<int64_t>(std::min<int64_t>(ceil_div(256, tiles), K / 256), 64);
+    while (S > 1 && (size_t)S * M * N * esz > splitk_bytes) --S;
+  }
+  g.kchunk = S > 1 ? ceil_div(ceil_div(K, S), 32) * 32 : K;
+  if (S > 1) S = (int)ceil_div(K, g.kchunk);
+  g.part = S > 1 ? splitk_ws : nullptr;
+  dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)S);
+  if (prec == P_F64)
+    hipLaunchKernelGGL(gemm_mfma_kernel<P_F64>, grid, dim3(NT), 0, st, g);
+  else if (prec == P_F32)
+    hipLaunchKernelGGL(gemm_mfma_kernel<P_F32>, grid, dim3(NT), 0, st, g);
+  else if (prec == P_BF16X3)
+    hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(NT), 0, st, g);
+  else {
+    set_error(TN_ERR_UNSUPPORTED, "contract: unknown precision %d", prec);
+    return TN_ERR_UNSUPPORTED;
+  }
+  TN_CHECK_LAUNCH();
+  if (S > 1) {
+    unsigned blocks = (unsigned)ceil_div(M * N, 256);
+    if (dtype == TN_F64)
+      hipLaunchKernelGGL(splitk_reduce<double>, dim3(blocks), dim3(256), 0, st,
+                         (const double*)splitk_ws, S, M, N, g.cm, g.cn, (double*)C, alpha, beta);
+    else
+      hipLaunchKernelGGL(splitk_reduce<float>, dim3(blocks), dim3(256), 0, st,
+                         (const float*)splitk_ws, S, M, N, g.cm, g.cn, (float*)C, alpha, beta);
+    TN_CHECK_LAUNCH();
+  }
+  return TN_OK;
+}
+
+Modes mk(std::initializer_list<int64_t> dims, std::initializer_list<int64_t> strides) {
+  Modes m{};
+  m.n = (int)dims.size();
+  int i = 0;
+  for (auto d : dims) m.dims[i++] = d;
+  i = 0;
+  for (auto s : strides) m.strides[i++] = s;
+  for (int j = m.n; j < TN_MAX_MODES; ++j) { m.dims[j] = 1; m.strides[j] = 0; }
+  return m;
+}
+
+}  // namespace tn
+
+using namespace tn;
+
+extern "C" int tn_contract(const void* A, const tn_modes* a_m, const tn_modes* a_k,
+                           const void* B, const tn_modes* b_k, const tn_modes* b_n, void* C,
+                           const tn_modes* c_m, const tn_modes* c_n, int dtype, int precision,
+                           double alpha, double beta, void* stream) {
+  for (const tn_modes* m : {a_m, a_k, b_k, b_n, c_m, c_n})
+    TN_REQUIRE(m == nullptr || (m->nmodes >= 0 && m->nmodes <= TN_MAX_MODES),
+               "tn_contract: mode group has too many modes");
+  TN_REQUIRE(dims_equal(a_m, c_m), "tn_contract: A row modes and C row modes differ");
+  TN_REQUIRE(dims_equal(b_n, c_n), "tn_contract: B col modes and C col modes differ");
+  TN_REQUIRE(dims_equal(a_k, b_k), "tn_contract: contracted modes of A and B differ");
+  Modes am = to_modes(a_m), ak = to_modes(a_k), bk = to_modes(b_k), bn = to_modes(b_n),
+        cm = to_modes(c_m), cn = to_modes(c_n);
+  for (const Modes* m : {&am, &ak, &bn})
+    for (int i = 0; i < m->n; ++i) TN_REQUIRE(m->dims[i] >= 0, "tn_contract: negative dimension");
+  return contract_launch(A, am, ak, B, bk, bn, C, cm, cn, dtype, precision, alpha, beta, nullptr,
+                         0, as_stream(stream));
+}

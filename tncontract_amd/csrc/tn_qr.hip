@@ -1,0 +1,532 @@
+// Householder QR with FP64 reflectors (replaces np.linalg.qr(mode="reduced"),
+// i.e. LAPACK geqrf + orgqr, called by tensor_qr at tensor.py:1044).
+//
+// Layout: the m x n input is gathered once into a column-major workspace W
+// (ld = m) so every reflector is a contiguous column.  Two-level blocking:
+//  * inner panels of w <= 8 columns are factored by ONE 1024-thread
+//    workgroup with the whole panel held in registers (R rows per thread),
+//    so the two reductions per column are LDS/shuffle reductions, not grid
+//    synchronisations;  the panel also forms its w x w compact-WY factor T;
+//  * the remaining columns of the 64-wide outer block are updated by a
+//    column-parallel kernel, and the trailing matrix by three MFMA GEMMs
+//    (Y = V^T A, Y = T^T Y, A -= V Y) with split-K for the first.
+// Q is formed explicitly (orgqr order: blocks applied last to first to
+// [I; 0]).  Sign convention as LAPACK dlarfg: beta = -sign(alpha)*norm.
+#include "tn_internal.h"
+
+#include <vector>
+
+namespace tn {
+
+constexpr int QR_NB = 64;
+constexpr int PT = 1024;  // panel threads
+constexpr int NWAVE = PT / 64;
+
+// Block-wide sum of nv (<= NV) values; every thread receives the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], int nv, double* lds, double* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    if (q < nv) v[q] = wave_sum(v[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      if (q < nv) lds[wave * NV + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) s += lds[w * NV + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    if (q < nv) v[q] = out[q];
+  __syncthreads();
+}
+
+// Factor columns [c0, c0+w) (w <= W) over rows [c0, m) of W (column-major).
+template <int R, int W>
+__global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, int64_t m,
+                                                      int64_t ld, int64_t c0, int w,
+                                                      double* __restrict__ tau,
+                                                      double* __restrict__ Tp) {
+  constexpr int NP = W * (W - 1) / 2 > W ? W * (W - 1) / 2 : W;
+  __shared__ double lds[NWAVE * NP];
+  __shared__ double out[NP];
+  __shared__ double piv[W];  // pivot value of column j (one slot per column: no reuse race)
+  __shared__ double taus[W];
+  const int t = threadIdx.x;
+  double x[R][W];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    int64_t r = c0 + t + (int64_t)PT * i;
+#pragma unroll
+    for (int j = 0; j < W; ++j) x[i][j] = (r < m && j < w) ? A[r + (c0 + j) * ld] : 0.0;
+  }
+  for (int j = 0; j < w; ++j) {
+    const int64_t p = c0 + j;  // pivot row, owned by thread j (i = 0)
+    double s[1] = {0.0};
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      int64_t r = c0 + t + (int64_t)PT * i;
+      if (r > p && r < m) s[0] = fma(x[i][j], x[i][j], s[0]);
+    }
+    if (t == j) piv[j] = x[0][j];
+    block_sum<1>(s, 1, lds, out);
+    const double alpha = piv[j];
+    const double xnorm2 = s[0];
+    double tj = 0.0, scale = 0.0, beta = alpha;
+    if (xnorm2 > 0.0) {
+      double nrm = sqrt(alpha * alpha + xnorm2);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      tj = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    // v below the pivot, beta on the diagonal
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      int64_t r = c0 + t + (int64_t)PT * i;
+      if (r > p && r < m) x[i][j] *= scale;
+    }
+    if (t == j) x[0][j] = beta;
+    if (t == 0) taus[j] = tj;
+    // apply H_j = I - tau v v^T to the remaining panel columns
+    const int nrest = w - 1 - j;
+    if (nrest > 0 && tj != 0.0) {
+      double d[W];
+#pragma unroll
+      for (int q = 0; q < W; ++q) d[q] = 0.0;
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        int64_t r = c0 + t + (int64_t)PT * i;
+        if (r > p && r < m) {
+#pragma unroll
+          for (int q = 0; q < W; ++q)
+            if (q > j) d[q] = fma(x[i][j], x[i][q], d[q]);
+        }
+      }
+      // pivot-row contribution (v_p = 1)
+      if (t == j) {
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+          if (q > j) d[q] += x[0][q];
+      }
+      double dd[W];
+#pragma unroll
+      for (int q = 0; q < W; ++q) dd[q] = d[q];
+      block_sum<W>(dd, w, lds, out);
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        int64_t r = c0 + t + (int64_t)PT * i;
+        if (r >= p && r < m) {
+          double v = (r == p) ? 1.0 : x[i][j];
+#pragma unroll
+          for (int q = 0; q < W; ++q)
+            if (q > j && q < w) x[i][q] -= tj * dd[q] * v;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // compact-WY factor of the panel: G[a][b] = v_a . v_b (a < b)
+  double g[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) g[q] = 0.0;
+  {
+    int q = 0;
+#pragma unroll
+    for (int a = 0; a < W; ++a)
+#pragma unroll
+      for (int b = a + 1; b < W; ++b, ++q) {
+        if (b >= w) continue;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          int64_t r = c0 + t + (int64_t)PT * i;
+          if (r < m) {
+            double va = (r == c0 + a) ? 1.0 : (r > c0 + a ? x[i][a] : 0.0);
+            double vb = (r == c0 + b) ? 1.0 : (r > c0 + b ? x[i][b] : 0.0);
+            g[q] = fma(va, vb, g[q]);
+          }
+        }
+      }
+  }
+  block_sum<NP>(g, W * (W - 1) / 2, lds, out);
+  if (t == 0) {
+    double T[W][W];
+#pragma unroll
+    for (int a = 0; a < W; ++a)
+#pragma unroll
+      for (int b = 0; b < W; ++b) T[a][b] = 0.0;
+    int q0[W][W];
+    int q = 0;
+    for (int a = 0; a < W; ++a)
+      for (int b = a + 1; b < W; ++b) q0[a][b] = q++;
+    for (int b = 0; b < w; ++b) {
+      T[b][b] = taus[b];
+      for (int a = 0; a < b; ++a) {
+        double acc = 0.0;
+        for (int c = a; c < b; ++c) acc += T[a][c] * g[q0[c][b]];
+        T[a][b] = -taus[b] * acc;
+      }
+    }
+    for (int a = 0; a < w; ++a) {
+      tau[c0 + a] = taus[a];
+      for (int b = 0; b < w; ++b) Tp[a * w + b] = T[a][b];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    int64_t r = c0 + t + (int64_t)PT * i;
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      if (r < m && j < w) A[r + (c0 + j) * ld] = x[i][j];
+  }
+}
+
+// W[c0:m, c] = (I - V T^T V^T) W[c0:m, c] for c in [cb, ce): one block per column.
+template <int W>
+__global__ void __launch_bounds__(256) qr_apply_panel_kernel(double* __restrict__ A, int64_t m,
+                                                             int64_t ld, int64_t c0, int w,
+                                                             const double* __restrict__ Tp,
+                                                             int64_t cb) {
+  __shared__ double red[4 * W];
+  __shared__ double z[W];
+  const int64_t c = cb + blockIdx.x;
+  double y[W];
+#pragma unroll
+  for (int a = 0; a < W; ++a) y[a] = 0.0;
+  for (int64_t r = c0 + threadIdx.x; r < m; r += 256) {
+    double v = A[r + c * ld];
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+      if (a < w) {
+        double va = (r == c0 + a) ? 1.0 : (r > c0 + a ? A[r + (c0 + a) * ld] : 0.0);
+        y[a] = fma(va, v, y[a]);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < W; ++a) y[a] = wave_sum(y[a]);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int a = 0; a < W; ++a) red[(threadIdx.x >> 6) * W + a] = y[a];
+  __syncthreads();
+  if (threadIdx.x < w) {
+    // z = T^T y, T upper triangular
+    int b = threadIdx.x;
+    double acc = 0.0;
+    for (int a = 0; a <= b; ++a) {
+      double ya = red[a] + red[W + a] + red[2 * W + a] + red[3 * W + a];
+      acc += Tp[a * w + b] * ya;
+    }
+    z[b] = acc;
+  }
+  __syncthreads();
+  for (int64_t r = c0 + threadIdx.x; r < m; r += 256) {
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+      if (a < w) {
+        double va = (r == c0 + a) ? 1.0 : (r > c0 + a ? A[r + (c0 + a) * ld] : 0.0);
+        acc = fma(va, z[a], acc);
+      }
+    }
+    A[r + c * ld] -= acc;
+  }
+}
+
+// Explicit V (mr x nb, column-major, ld = mr) of the outer block at j0.
+__global__ void qr_build_v_kernel(const double* __restrict__ A, int64_t m, int64_t ld,
+                                  int64_t j0, int nb, double* __restrict__ V) {
+  const int64_t mr = m - j0;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= mr * nb) return;
+  int64_t a = i / mr, r = i - a * mr;
+  V[i] = (r == a) ? 1.0 : (r > a ? A[(j0 + r) + (j0 + a) * ld] : 0.0);
+}
+
+// T (nb x nb, row-major) from G = V^T V and tau (LAPACK dlarft, forward/columnwise).
+__global__ void __launch_bounds__(256) qr_larft_kernel(const double* __restrict__ G, int nb,
+                                                       const double* __restrict__ tau,
+                                                       double* __restrict__ T) {
+  __shared__ double Ts[QR_NB * QR_NB];
+  for (int i = threadIdx.x; i < nb * nb; i += 256) Ts[i] = 0.0;
+  __syncthreads();
+  for (int b = 0; b < nb; ++b) {
+    const double tb = tau[b];
+    if (threadIdx.x < b) {
+      int a = threadIdx.x;
+      double acc = 0.0;
+      for (int c = a; c < b; ++c) acc += Ts[a * nb + c] * G[c * nb + b];
+      Ts[a * nb + b] = -tb * acc;
+    }
+    if (threadIdx.x == 0) Ts[b * nb + b] = tb;
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < nb * nb; i += 256) T[i] = Ts[i];
+}
+
+__global__ void qr_init_q_kernel(double* __restrict__ Q, int64_t m, int64_t k) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m * k) return;
+  int64_t c = i / m, r = i - c * m;
+  Q[i] = (r == c) ? 1.0 : 0.0;
+}
+
+// R (k x n row-major) = upper triangle of W.
+template <typename T>
+__global__ void qr_extract_r_kernel(const double* __restrict__ A, int64_t m, int64_t k, int64_t n,
+                                    T* __restrict__ R) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k * n) return;
+  int64_t r = i / n, c = i - r * n;
+  R[i] = (T)(c >= r ? A[r + c * m] : 0.0);
+}
+
+template <typename T>
+__global__ void colmajor_to_rowmajor_kernel(const double* __restrict__ A, int64_t m, int64_t k,
+                                            T* __restrict__ Q) {
+  __shared__ double tile[32][33];
+  int64_t bx = (int64_t)blockIdx.x * 32, by = (int64_t)blockIdx.y * 32;
+  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    int64_t r = bx + tx, c = by + j;
+    if (r < m && c < k) tile[j][tx] = A[r + c * m];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    int64_t r = bx + j, c = by + tx;
+    if (r < m && c < k) Q[r * k + c] = (T)tile[tx][j];
+  }
+}
+
+// gather: W[i + j*m] = A(row i, col j) for any strided input (T -> double)
+template <typename T>
+__global__ void gather_colmajor_kernel(const T* __restrict__ src, Modes rows, Modes cols,
+                                       int64_t m, int64_t n, double* __restrict__ W) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < m * n; i += stride) {
+    int64_t c = i / m, r = i - c * m;
+    W[i] = (double)src[mode_offset(rows, r) + mode_offset(cols, c)];
+  }
+}
+
+struct QrWs {
+  double *W, *tau, *V, *Tall, *G, *Y, *Y2, *Tp, *Qw, *part;
+  size_t part_bytes;
+};
+
+static size_t qr_layout(int64_t m, int64_t n, char* base, size_t cap, QrWs* ws) {
+  const int64_t k = std::min(m, n);
+  const int64_t nblk = ceil_div(std::max<int64_t>(k, 1), QR_NB);
+  const int64_t wn = std::max(n, k);
+  Arena a{base, cap, 0};
+  QrWs w{};
+  w.W = a.take<double>((size_t)(m * n));
+  w.tau = a.take<double>((size_t)std::max<int64_t>(k, 1));
+  w.V = a.take<double>((size_t)(m * QR_NB));
+  w.Tall = a.take<double>((size_t)(nblk * QR_NB * QR_NB));
+  w.G = a.take<double>((size_t)(QR_NB * QR_NB));
+  w.Y = a.take<double>((size_t)(QR_NB * wn));
+  w.Y2 = a.take<double>((size_t)(QR_NB * wn));
+  w.Tp = a.take<double>(64);
+  w.Qw = a.take<double>((size_t)(m * k));
+  w.part_bytes = (size_t)16 * QR_NB * wn * sizeof(double);
+  w.part = a.take<double>((size_t)16 * QR_NB * wn);
+  if (ws) *ws = w;
+  return a.used;
+}
+
+typedef void (*PanelFn)(double*, int64_t, int64_t, int64_t, int, double*, double*);
+
+struct PanelChoice {
+  int R, W;
+  const void* fn;
+};
+
+template <int R, int W>
+static int launch_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, double* tau,
+                        double* Tp, hipStream_t st) {
+  hipLaunchKernelGGL((qr_panel_kernel<R, W>), dim3(1), dim3(PT), 0, st, A, m, ld, c0, w, tau, Tp);
+  TN_CHECK_LAUNCH();
+  return TN_OK;
+}
+
+// Pick the register-panel instantiation for m_rem rows; returns panel width.
+static int panel_width(int64_t mrem) {
+  int64_t R = ceil_div(mrem, PT);
+  if (R <= 6) return 8;
+  if (R <= 12) return 4;
+  if (R <= 16) return 2;
+  return 1;
+}
+
+static int run_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, double* tau, double* Tp,
+                     hipStream_t st) {
+  int64_t R = ceil_div(m - c0, PT);
+  if (R <= 1) return launch_panel<1, 8>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 2) return launch_panel<2, 8>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 3) return launch_panel<3, 8>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 4) return launch_panel<4, 8>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 5) return launch_panel<5, 8>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 6) return launch_panel<6, 8>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 8) return launch_panel<8, 4>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 10) return launch_panel<10, 4>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 12) return launch_panel<12, 4>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 16) return launch_panel<16, 2>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 24) return launch_panel<24, 1>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 32) return launch_panel<32, 1>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 48) return launch_panel<48, 1>(A, m, ld, c0, w, tau, Tp, st);
+  set_error(TN_ERR_UNSUPPORTED, "tn_qr: %lld rows exceed the register-panel limit (49152)",
+            (long long)(m - c0));
+  return TN_ERR_UNSUPPORTED;
+}
+
+static int run_apply_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, const double* Tp,
+                           int64_t cb, int64_t ce, hipStream_t st) {
+  if (ce <= cb) return TN_OK;
+  unsigned blocks = (unsigned)(ce - cb);
+  if (w > 4)
+    hipLaunchKernelGGL(qr_apply_panel_kernel<8>, dim3(blocks), dim3(256), 0, st, A, m, ld, c0, w,
+                       Tp, cb);
+  else if (w > 2)
+    hipLaunchKernelGGL(qr_apply_panel_kernel<4>, dim3(blocks), dim3(256), 0, st, A, m, ld, c0, w,
+                       Tp, cb);
+  else
+    hipLaunchKernelGGL(qr_apply_panel_kernel<2>, dim3(blocks), dim3(256), 0, st, A, m, ld, c0, w,
+                       Tp, cb);
+  TN_CHECK_LAUNCH();
+  return TN_OK;
+}
+
+// Factor W (m x n column-major, ld=m) in place; fills tau and the per-block T.
+int qr_factor(double* W, int64_t m, int64_t n, const QrWs& ws, hipStream_t st) {
+  const int64_t k = std::min(m, n);
+  for (int64_t j0 = 0; j0 < k; j0 += QR_NB) {
+    const int nbo = (int)std::min<int64_t>(QR_NB, k - j0);
+    // inner panels inside the outer block
+    for (int64_t jj = j0; jj < j0 + nbo;) {
+      int wmax = panel_width(m - jj);
+      int w = (int)std::min<int64_t>(wmax, j0 + nbo - jj);
+      TN_TRY(run_panel(W, m, m, jj, w, ws.tau, ws.Tp, st));
+      TN_TRY(run_apply_panel(W, m, m, jj, w, ws.Tp, jj + w, j0 + nbo, st));
+      jj += w;
+    }
+    // compact-WY for the whole outer block (also used to form Q)
+    const int64_t mr = m - j0;
+    hipLaunchKernelGGL(qr_build_v_kernel, dim3((unsigned)ceil_div(mr * nbo, 256)), dim3(256), 0,
+                       st, W, m, m, j0, nbo, ws.V);
+    TN_CHECK_LAUNCH();
+    // G = V^T V
+    TN_TRY(contract_launch(ws.V, mk({nbo}, {mr}), mk({mr}, {1}), ws.V, mk({mr}, {1}),
+                           mk({nbo}, {mr}), ws.G, mk({nbo}, {nbo}), mk({nbo}, {1}), TN_F64,
+                           TN_PREC_NATIVE, 1.0, 0.0, ws.part, ws.part_bytes, st));
+    double* T = ws.Tall + (j0 / QR_NB) * QR_NB * QR_NB;
+    hipLaunchKernelGGL(qr_larft_kernel, dim3(1), dim3(256), 0, st, ws.G, nbo, ws.tau + j0, T);
+    TN_CHECK_LAUNCH();
+    const int64_t nt = n - (j0 + nbo);
+    if (nt > 0) {
+      double* At = W + j0 + (j0 + nbo) * m;
+      // Y = V^T At   (nbo x nt)
+      TN_TRY(contract_launch(ws.V, mk({nbo}, {mr}), mk({mr}, {1}), At, mk({mr}, {1}),
+                             mk({nt}, {m}), ws.Y, mk({nbo}, {nt}), mk({nt}, {1}), TN_F64,
+                             TN_PREC_NATIVE, 1.0, 0.0, ws.part, ws.part_bytes, st));
+      // Y2 = T^T Y
+      TN_TRY(contract_launch(T, mk({nbo}, {1}), mk({nbo}, {nbo}), ws.Y, mk({nbo}, {nt}),
+                             mk({nt}, {1}), ws.Y2, mk({nbo}, {nt}), mk({nt}, {1}), TN_F64,
+                             TN_PREC_NATIVE, 1.0, 0.0, nullptr, 0, st));
+      // At -= V Y2
+      TN_TRY(contract_launch(ws.V, mk({mr}, {1}), mk({nbo}, {mr}), ws.Y2, mk({nbo}, {nt}),
+                             mk({nt}, {1}), At, mk({mr}, {1}), mk({nt}, {m}), TN_F64,
+                             TN_PREC_NATIVE, -1.0, 1.0, nullptr, 0, st));
+    }
+  }
+  return TN_OK;
+}
+
+// Qw (m x k column-major) = H_1 ... H_k [I; 0]
+int qr_form_q(const double* W, int64_t m, int64_t n, const QrWs& ws, double* Qw, hipStream_t st) {
+  const int64_t k = std::min(m, n);
+  hipLaunchKernelGGL(qr_init_q_kernel, dim3((unsigned)ceil_div(m * k, 256)), dim3(256), 0, st, Qw,
+                     m, k);
+  TN_CHECK_LAUNCH();
+  const int64_t nblk = ceil_div(k, QR_NB);
+  for (int64_t b = nblk - 1; b >= 0; --b) {
+    const int64_t j0 = b * QR_NB;
+    const int nbo = (int)std::min<int64_t>(QR_NB, k - j0);
+    const int64_t mr = m - j0, nc = k - j0;
+    hipLaunchKernelGGL(qr_build_v_kernel, dim3((unsigned)ceil_div(mr * nbo, 256)), dim3(256), 0,
+                       st, W, m, m, j0, nbo, ws.V);
+    TN_CHECK_LAUNCH();
+    const double* T = ws.Tall + b * QR_NB * QR_NB;
+    double* Qs = Qw + j0 + j0 * m;
+    TN_TRY(contract_launch(ws.V, mk({nbo}, {mr}), mk({mr}, {1}), Qs, mk({mr}, {1}),
+                           mk({nc}, {m}), ws.Y, mk({nbo}, {nc}), mk({nc}, {1}), TN_F64,
+                           TN_PREC_NATIVE, 1.0, 0.0, ws.part, ws.part_bytes, st));
+    TN_TRY(contract_launch(T, mk({nbo}, {nbo}), mk({nbo}, {1}), ws.Y, mk({nbo}, {nc}),
+                           mk({nc}, {1}), ws.Y2, mk({nbo}, {nc}), mk({nc}, {1}), TN_F64,
+                           TN_PREC_NATIVE, 1.0, 0.0, nullptr, 0, st));
+    TN_TRY(contract_launch(ws.V, mk({mr}, {1}), mk({nbo}, {mr}), ws.Y2, mk({nbo}, {nc}),
+                           mk({nc}, {1}), Qs, mk({mr}, {1}), mk({nc}, {m}), TN_F64,
+                           TN_PREC_NATIVE, -1.0, 1.0, nullptr, 0, st));
+  }
+  return TN_OK;
+}
+
+size_t qr_workspace_bytes(int64_t m, int64_t n) { return qr_layout(m, n, nullptr, 0, nullptr) + 256; }
+
+}  // namespace tn
+
+using namespace tn;
+
+extern "C" size_t tn_qr_workspace(int64_t m, int64_t n, int dtype) {
+  (void)dtype;
+  if (m <= 0 || n <= 0) return 256;
+  return tn::qr_workspace_bytes(m, n);
+}
+
+extern "C" int tn_qr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m,
+                     int64_t n, void* Q, void* R, int dtype, void* ws, size_t ws_bytes,
+                     void* stream) {
+  TN_REQUIRE(m >= 0 && n >= 0, "tn_qr: negative size");
+  TN_REQUIRE(dtype == TN_F64 || dtype == TN_F32, "tn_qr: unsupported dtype %d", dtype);
+  Modes rows = to_modes(a_rows), cols = to_modes(a_cols);
+  TN_REQUIRE(modes_size(rows) == m && modes_size(cols) == n, "tn_qr: modes do not match m x n");
+  const int64_t k = std::min(m, n);
+  if (k == 0) return TN_OK;
+  TN_REQUIRE(A && Q && R, "tn_qr: null pointer");
+  TN_REQUIRE(ws && ws_bytes >= tn_qr_workspace(m, n, dtype), "tn_qr: workspace too small");
+  hipStream_t st = as_stream(stream);
+  QrWs w{};
+  qr_layout(m, n, (char*)ws, ws_bytes, &w);
+  unsigned gb = (unsigned)std::min<int64_t>(ceil_div(m * n, 256), 65536);
+  if (dtype == TN_F64)
+    hipLaunchKernelGGL(gather_colmajor_kernel<double>, dim3(gb), dim3(256), 0, st,
+                       (const double*)A, rows, cols, m, n, w.W);
+  else
+    hipLaunchKernelGGL(gather_colmajor_kernel<float>, dim3(gb), dim3(256), 0, st,
+                       (const float*)A, rows, cols, m, n, w.W);
+  TN_CHECK_LAUNCH();
+  TN_TRY(qr_factor(w.W, m, n, w, st));
+  TN_TRY(qr_form_q(w.W, m, n, w, w.Qw, st));
+  dim3 tg((unsigned)ceil_div(m, 32), (unsigned)ceil_div(k, 32));
+  unsigned rb = (unsigned)ceil_div(k * n, 256);
+  if (dtype == TN_F64) {
+    hipLaunchKernelGGL(colmajor_to_rowmajor_kernel<double>, tg, dim3(256), 0, st, w.Qw, m, k,
+                       (double*)Q);
+    hipLaunchKernelGGL(qr_extract_r_kernel<double>, dim3(rb), dim3(256), 0, st, w.W, m, k, n,
+                       (double*)R);
+  } else {
+    hipLaunchKernelGGL(colmajor_to_rowmajor_kernel<float>, tg, dim3(256), 0, st, w.Qw, m, k,
+                       (float*)Q);
+    hipLaunchKernelGGL(qr_extract_r_kernel<float>, dim3(rb), dim3(256), 0, st, w.W, m, k, n,
+                       (float*)R);
+  }
+  TN_CHECK_LAUNCH();
+  return TN_OK;
+}

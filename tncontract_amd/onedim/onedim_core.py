@@ -1,0 +1,444 @@
+"""Matrix product states/operators on the GPU (hot subset of
+tncontract/onedim/onedim_core.py).
+
+Same classes, methods, labels, axis orders and in-place mutation semantics
+as the reference; every tensor is a device ``Tensor`` and every contraction,
+QR, SVD, norm and scale runs in libtncontract_amd.  The sweeps stay in Python
+(one C-ABI call per factorisation / contraction); the only host reads are the
+singular values needed for truncation decisions and the scalar norms the
+reference itself reads.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from tncontract_amd import _lib as L
+from tncontract_amd import tensor as tsr
+from tncontract_amd.label import unique_label
+
+__all__ = ["MatrixProductState", "MatrixProductOperator", "OneDimensionalTensorNetwork",
+           "check_canonical_form_mps", "contract_mps_mpo", "frob_distance_squared",
+           "inner_product_mps", "ladder_contract", "left_canonical_form_mps",
+           "mps_complex_conjugate", "reverse_mps", "right_canonical_form_mps",
+           "svd_compress_mps", "contract_virtual_indices"]
+
+
+class OneDimensionalTensorNetwork:
+    """onedim_core.py:31-183"""
+
+    def __init__(self, tensors, left_label="left", right_label="right"):
+        self.left_label = left_label
+        self.right_label = right_label
+        self.data = [x.copy() for x in tensors]
+        for x in self.data:
+            if left_label not in x.labels:
+                x.add_dummy_index(left_label)
+            if right_label not in x.labels:
+                x.add_dummy_index(right_label)
+
+    def __iter__(self):
+        return iter(self.data)
+
+    def __len__(self):
+        return len(self.data)
+
+    def __getitem__(self, key):
+        return self.data[key]
+
+    def __setitem__(self, key, value):
+        self.data[key] = value
+
+    def copy(self):
+        return OneDimensionalTensorNetwork([x.copy() for x in self], self.left_label,
+                                           self.right_label)
+
+    def reverse(self):
+        self.data = self.data[::-1]
+        self.left_label, self.right_label = self.right_label, self.left_label
+
+    def complex_conjugate(self):
+        for x in self.data:
+            x.conjugate()
+
+    def replace_labels(self, old_labels, new_labels):
+        if not isinstance(old_labels, list):
+            old_labels = [old_labels]
+        if not isinstance(new_labels, list):
+            new_labels = [new_labels]
+        for x in self.data:
+            x.replace_label(old_labels, new_labels)
+        if self.left_label in old_labels:
+            self.left_label = new_labels[old_labels.index(self.left_label)]
+        if self.right_label in old_labels:
+            self.right_label = new_labels[old_labels.index(self.right_label)]
+
+    def standard_virtual_labels(self, suffix=""):
+        self.replace_labels([self.left_label, self.right_label],
+                            ["left" + suffix, "right" + suffix])
+
+    def unique_virtual_labels(self):
+        self.replace_labels([self.left_label, self.right_label], [unique_label(), unique_label()])
+
+    def leftdim(self, site):
+        return self.data[site].index_dimension(self.left_label)
+
+    def rightdim(self, site):
+        return self.data[site].index_dimension(self.right_label)
+
+    def bonddims(self):
+        if self.nsites == 0:
+            return []
+        return [self.leftdim(0)] + [self.rightdim(i) for i in range(self.nsites)]
+
+    @property
+    def nsites(self):
+        return len(self.data)
+
+    @property
+    def nsites_physical(self):
+        return self.nsites
+
+
+class MatrixProductState(OneDimensionalTensorNetwork):
+    """onedim_core.py:186-660 (hot subset)."""
+
+    def __init__(self, tensors, left_label="left", right_label="right", phys_label="phys"):
+        OneDimensionalTensorNetwork.__init__(self, tensors, left_label, right_label)
+        self.phys_label = phys_label
+
+    def __repr__(self):
+        return ("MatrixProductState(sites=%d, left_label=%r, right_label=%r, phys_label=%r)"
+                % (len(self), self.left_label, self.right_label, self.phys_label))
+
+    def copy(self):
+        return MatrixProductState([x.copy() for x in self], self.left_label, self.right_label,
+                                  self.phys_label)
+
+    def replace_labels(self, old_labels, new_labels):
+        if not isinstance(old_labels, list):
+            old_labels = [old_labels]
+        if not isinstance(new_labels, list):
+            new_labels = [new_labels]
+        OneDimensionalTensorNetwork.replace_labels(self, old_labels, new_labels)
+        if self.phys_label in old_labels:
+            self.phys_label = new_labels[old_labels.index(self.phys_label)]
+
+    def standard_labels(self, suffix=""):
+        self.replace_labels([self.left_label, self.right_label, self.phys_label],
+                            ["left" + suffix, "right" + suffix, "phys" + suffix])
+
+    def physdim(self, site):
+        return self.data[site].index_dimension(self.phys_label)
+
+    def physical_site(self, n):
+        return n
+
+    def _zero_product_state(self):
+        # onedim_core.py:274-278 / 305-309 / 323-330
+        for k in range(len(self)):
+            d = self[k].index_dimension(self.phys_label)
+            self[k] = tsr.Tensor(np.zeros((d, 1, 1)), [self.phys_label, self.left_label,
+                                                      self.right_label])
+
+    def left_canonise(self, start=0, end=-1, chi=None, threshold=1e-14, normalise=False,
+                      qr_decomposition=False):
+        """onedim_core.py:218-358"""
+        N = len(self)
+        if end == -1:
+            end = N
+        if qr_decomposition:
+            for i in range(start, end):
+                if i == N - 1:
+                    norm = self[i].norm()
+                    if norm == 0.0:
+                        self._zero_product_state()
+                        return
+                    if normalise and start == 0:
+                        self[i].scale_(1.0 / norm)
+                    return
+                qr_label = unique_label()
+                Q, R = tsr.tensor_qr(self[i], [self.phys_label, self.left_label], qr_label=qr_label)
+                Q.replace_label(qr_label + "in", self.right_label)
+                self[i] = Q
+                self[i + 1] = tsr.contract(R, self[i + 1], self.right_label, self.left_label)
+                self[i + 1].replace_label(qr_label + "out", self.left_label)
+            return
+        norm = 1.0
+        for i in range(start, end):
+            if i == N - 1:
+                nrm = self[i].norm()
+                if nrm == 0.0:
+                    self._zero_product_state()
+                    return
+                if normalise and start == 0:
+                    self[i].scale_(1.0 / nrm)
+                else:
+                    self[i].scale_(norm)
+                return
+            svd_label = unique_label()
+            U, S, V, s = tsr._tensor_svd(self[i], [self.phys_label, self.left_label],
+                                         svd_label=svd_label)
+            sv = s.cpu().numpy()
+            s0 = sv[0] if len(sv) else 0.0
+            if s0 == 0.0:
+                self._zero_product_state()
+                return
+            sv = sv / s0
+            norm *= s0
+            keep = sv[sv > threshold]
+            if chi:
+                keep = keep[:chi]
+            k = len(keep)
+            S = tsr.Tensor._wrap(L.diag_embed(_upload(keep), U.dtype), S.labels)
+            U._data = U.data[:, :, :k]
+            V._data = V.data[:k]
+            U.replace_label(svd_label + "in", self.right_label)
+            self[i] = U
+            nxt = tsr.contract(V, self[i + 1], self.right_label, self.left_label)
+            nxt = tsr.contract(S, nxt, [svd_label + "in"], [svd_label + "out"])
+            nxt.replace_label(svd_label + "out", self.left_label)
+            self[i + 1] = nxt
+            if i == end - 1:
+                self[i + 1].scale_(norm)
+
+    def right_canonise(self, start=0, end=-1, chi=None, threshold=1e-14, normalise=False,
+                       qr_decomposition=False):
+        """onedim_core.py:360-378"""
+        self.reverse()
+        N = len(self)
+        if end == -1:
+            end = N
+        self.left_canonise(start=N - end, end=N - start, chi=chi, threshold=threshold,
+                           normalise=normalise, qr_decomposition=qr_decomposition)
+        self.reverse()
+
+    def norm(self, canonical_form=False):
+        """onedim_core.py:642-660"""
+        if canonical_form == "left":
+            return self[-1].norm()
+        if canonical_form == "right":
+            return self[0].norm()
+        return float(np.sqrt(inner_product_mps(self, self)))
+
+    def svd_compress(self, chi=None, threshold=1e-15, normalise=False, reverse=False):
+        """onedim_core.py:463-484: QR left sweep, normalise, truncating SVD
+        right sweep, restore the norm."""
+        if reverse:
+            self.reverse()
+        self.left_canonise(normalise=False, qr_decomposition=True)
+        norm = self.norm(canonical_form="left")
+        self[-1].scale_(1.0 / norm)
+        self.right_canonise(chi=chi, threshold=threshold, normalise=False)
+        if not normalise:
+            self[0].scale_(norm)
+        if reverse:
+            self.reverse()
+
+    def check_canonical_form(self, threshold=1e-14, print_output=True):
+        """onedim_core.py:408-461"""
+        first_not_left = len(self) - 1
+        for i in range(len(self) - 1):
+            I = tsr.contract(self[i], self[i], [self.phys_label, self.left_label],
+                             [self.phys_label, self.left_label])
+            if _dist_identity(I) > threshold:
+                first_not_left = i
+                break
+        first_not_right = 0
+        for i in range(len(self) - 1, 0, -1):
+            I = tsr.contract(self[i], self[i], [self.phys_label, self.right_label],
+                             [self.phys_label, self.right_label])
+            if _dist_identity(I) > threshold:
+                first_not_right = i
+                break
+        if print_output:
+            print("first site not left canonised: %d, first site not right canonised: %d"
+                  % (first_not_left, first_not_right))
+        return first_not_left, first_not_right
+
+
+def _upload(arr):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(L.device())
+
+
+def _dist_identity(I):
+    """||I - 1||_F for a square device matrix (FP64 reductions on the GPU)."""
+    n = I.shape[0]
+    eye = L.diag_embed(_upload(np.ones(n)), I.dtype)
+    return float(np.sqrt(L.sumsq(L.axpby(L.contiguous(I.data), eye, 1.0, -1.0))))
+
+
+class MatrixProductOperator(OneDimensionalTensorNetwork):
+    """onedim_core.py:1329-1367"""
+
+    def __init__(self, tensors, left_label="left", right_label="right", physout_label="physout",
+                 physin_label="physin"):
+        OneDimensionalTensorNetwork.__init__(self, tensors, left_label, right_label)
+        self.physout_label = physout_label
+        self.physin_label = physin_label
+
+    def physoutdim(self, site):
+        return self.data[site].index_dimension(self.physout_label)
+
+    def physindim(self, site):
+        return self.data[site].index_dimension(self.physin_label)
+
+
+def contract_virtual_indices(array_1d, start=0, end=None, periodic_boundaries=True):
+    """onedim_core.py:1399-1422"""
+    C = array_1d[start].copy()
+    for x in array_1d[start + 1:end]:
+        C = tsr.contract(C, x, array_1d.right_label, array_1d.left_label)
+    if periodic_boundaries:
+        C.contract_internal(array_1d.right_label, array_1d.left_label)
+    return C
+
+
+def left_canonical_form_mps(orig_mps, chi=0, threshold=1e-14, normalise=False):
+    mps = orig_mps.copy()
+    mps.left_canonise(chi=chi, threshold=threshold, normalise=normalise)
+    return mps
+
+
+def right_canonical_form_mps(orig_mps, chi=0, threshold=1e-14, normalise=False):
+    mps = orig_mps.copy()
+    mps.right_canonise(chi=chi, threshold=threshold, normalise=normalise)
+    return mps
+
+
+def reverse_mps(orig_mps):
+    mps = orig_mps.copy()
+    mps.reverse()
+    return mps
+
+
+def check_canonical_form_mps(mps, threshold=1e-14, print_output=True):
+    return mps.check_canonical_form(threshold=threshold, print_output=print_output)
+
+
+def svd_compress_mps(orig_mps, chi, threshold=1e-15, normalise=False):
+    """onedim_core.py:1468-1473"""
+    mps = left_canonical_form_mps(orig_mps, threshold=threshold, normalise=normalise)
+    return right_canonical_form_mps(mps, chi=chi, threshold=threshold, normalise=normalise)
+
+
+def mps_complex_conjugate(mps):
+    new = mps.copy()
+    for x in new.data:
+        x.conjugate()
+    return new
+
+
+def ladder_contract(array1, array2, label1, label2, start=0, end=None,
+                    complex_conjugate_array1=False, left_output_label="left",
+                    right_output_label="right"):
+    """onedim_core.py:1491-1663 (full-chain left-to-right contraction)."""
+    if end is None:
+        end = min(array1.nsites, array2.nsites) - 1
+    if start != 0:
+        raise NotImplementedError("ladder_contract: only start=0 is on the GPU path")
+    a1 = OneDimensionalTensorNetwork(list(array1), array1.left_label, array1.right_label)
+    a2 = OneDimensionalTensorNetwork(list(array2), array2.left_label, array2.right_label)
+    a1.unique_virtual_labels()
+    a2.unique_virtual_labels()
+    rung = unique_label()
+    a1.replace_labels(label1, rung)
+    a2.replace_labels(label2, rung)
+    C = tsr.contract(a1[0], a2[0], rung, rung)
+    for i in range(1, end + 1):
+        C.contract(a1[i], a1.right_label, a1.left_label)
+        C.contract(a2[i], [a2.right_label, rung], [a2.left_label, rung])
+    C.replace_label([a1.right_label, a2.right_label],
+                    [right_output_label + "1", right_output_label + "2"])
+    C.remove_all_dummy_indices()
+    return C
+
+
+def inner_product_mps(mps_bra, mps_ket, complex_conjugate_bra=True, return_whole_tensor=False):
+    """onedim_core.py:1666-1683 -- FP64 transfer-matrix sweep on the GPU."""
+    t = ladder_contract(mps_bra, mps_ket, mps_bra.phys_label, mps_ket.phys_label,
+                        complex_conjugate_array1=complex_conjugate_bra)
+    if return_whole_tensor:
+        return t
+    return t.numpy()
+
+
+def frob_distance_squared(mps1, mps2):
+    """onedim_core.py:1686-1688"""
+    ip = inner_product_mps
+    return ip(mps1, mps1) + ip(mps2, mps2) - 2 * np.real(ip(mps1, mps2))
+
+
+def contract_mps_mpo(mps, mpo):
+    """onedim_core.py:1691-1708.  The consolidation permute (labels sorted,
+    equal labels fused with the MPS axis slower) is fused into the store of
+    the contraction kernel: each site is written once, already in its final
+    (left, physout, right) layout."""
+    N = len(mps)
+    new_mps = []
+    for i in range(N):
+        t = _contract_consolidated(mps[i], mpo[i], mps.phys_label, mpo.physin_label)
+        new_mps.append(t)
+    return MatrixProductState._from_owned(new_mps, mps.left_label, mps.right_label,
+                                          mpo.physout_label)
+
+
+def _contract_consolidated(t1, t2, l1, l2):
+    """contract(t1, t2, l1, l2) followed by consolidate_indices(), computed as
+    one kernel whose store offsets encode the consolidated layout."""
+    import ctypes
+    import torch
+    ax1 = [i for i, x in enumerate(t1.labels) if x == l1]
+    ax2 = [i for i, x in enumerate(t2.labels) if x == l2]
+    free1 = [i for i in range(t1.rank) if i not in ax1]
+    free2 = [i for i in range(t2.rank) if i not in ax2]
+    labels = [t1.labels[i] for i in free1] + [t2.labels[i] for i in free2]
+    dims = [t1.shape[i] for i in free1] + [t2.shape[i] for i in free2]
+    # consolidated layout: simulate consolidate_indices on (labels, axis ids)
+    groups = [[i] for i in range(len(labels))]
+    labs = list(labels)
+    for p, lab in enumerate(sorted(set(labels))):
+        idx = [i for i in range(len(labs)) if labs[i] == lab]
+        rest = [i for i in range(len(labs)) if i not in idx]
+        fused = [a for i in idx for a in groups[i]]
+        groups = [groups[i] for i in rest[:p]] + [fused] + [groups[i] for i in rest[p:]]
+        labs = [labs[i] for i in rest[:p]] + [lab] + [labs[i] for i in rest[p:]]
+    order = [a for g in groups for a in g]
+    # stride of each uncontracted axis in the final dense layout
+    stride = [0] * len(dims)
+    acc = 1
+    for a in reversed(order):
+        stride[a] = acc
+        acc *= dims[a]
+    d1, d2 = tsr._common_dtype(t1.data, t2.data)
+    out = torch.empty(acc, dtype=d1.dtype, device=d1.device)
+    nf1 = len(free1)
+    am = L.tensor_modes(d1, free1)
+    ak = L.tensor_modes(d1, ax1)
+    bk = L.tensor_modes(d2, ax2)
+    bn = L.tensor_modes(d2, free2)
+    cm = L.modes(dims[:nf1], stride[:nf1])
+    cn = L.modes(dims[nf1:], stride[nf1:])
+    if len(ax1) != len(ax2) or any(t1.shape[a] != t2.shape[b] for a, b in zip(ax1, ax2)):
+        raise ValueError("contract_mps_mpo: physical dimensions do not match")
+    L.check(L.lib().tn_contract(L.ptr(d1), ctypes.byref(am), ctypes.byref(ak), L.ptr(d2),
+                                ctypes.byref(bk), ctypes.byref(bn), L.ptr(out), ctypes.byref(cm),
+                                ctypes.byref(cn), L.dtype_code(d1), 0, 1.0, 0.0, L.stream_ptr()))
+    shape = [int(np.prod([dims[a] for a in g])) for g in groups]
+    return tsr.Tensor._wrap(out.view(shape), labs)
+
+
+def _from_owned(cls, tensors, left_label, right_label, phys_label):
+    """Construct without the defensive per-site copy (tensors are fresh)."""
+    obj = cls.__new__(cls)
+    obj.left_label, obj.right_label, obj.phys_label = left_label, right_label, phys_label
+    obj.data = list(tensors)
+    for x in obj.data:
+        if left_label not in x.labels:
+            x.add_dummy_index(left_label)
+        if right_label not in x.labels:
+            x.add_dummy_index(right_label)
+    return obj
+
+
+MatrixProductState._from_owned = classmethod(_from_owned)
