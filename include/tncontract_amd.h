@@ -78,6 +78,13 @@ size_t tn_sumsq_workspace(int64_t n);
 int tn_sumsq(const void* x, int dtype, int64_t n, double* out, void* ws, size_t ws_bytes,
              void* stream);
 
+/* x /= ||x||_F in place, ||x|| computed in FP64 and stored at *norm_out
+ * (device double, may be NULL); x untouched when the norm is 0.  Keeps the
+ * left environments of the structured sweep O(1) without a host round trip. */
+size_t tn_normalize_workspace(int64_t n);
+int tn_normalize(void* x, int dtype, int64_t n, double* norm_out, void* ws, size_t ws_bytes,
+                 void* stream);
+
 /* Internal contraction (np.trace, tensor.py:318-338):
  * dst[o] = sum_{t<diag_dim} src[off(out_modes, o) + t * diag_stride]. */
 int tn_trace(const void* src, const tn_modes* out_modes, int64_t diag_dim, int64_t diag_stride,
@@ -107,12 +114,24 @@ int tn_qr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t
  * Thin SVD of the m x n matrix A (decoded as in tn_qr): U m x k row-major,
  * S k (FP64, descending), Vt k x n row-major, k = min(m,n).  One-sided
  * block Jacobi with FP64 rotations.  tol: orthogonality tolerance (<=0 ->
- * 1e-14); max_sweeps <= 0 -> 40.  *sweeps_out (host, may be NULL) receives
+ * max(m,n) * eps, as LAPACK dgesvj); max_sweeps <= 0 -> 40.  *sweeps_out (host, may be NULL) receives
  * the number of sweeps used.  Returns TN_ERR_NOCONV if not converged. */
 size_t tn_svd_workspace(int64_t m, int64_t n, int dtype);
 int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m, int64_t n,
            void* U, double* S, void* Vt, int dtype, double tol, int max_sweeps, int* sweeps_out,
            void* ws, size_t ws_bytes, void* stream);
+
+/* ---- launch timing (no reference counterpart: measurement support) ------
+ * tn_prof_enable(1) makes the library record a HIP event pair, on the launch
+ * stream, around every launch of the instrumented kernels; tn_prof_collect
+ * synchronises those events and returns, for one tag, the summed elapsed
+ * time, launch count and the algorithmic flops / bytes the launcher assigned
+ * to each launch.  Tags: 0 FP64 GEMM, 1 FP32 GEMM, 2 BF16x3 GEMM, 3 Jacobi
+ * Gram, 4 Jacobi pair eigensolve, 5 Jacobi apply, 6 QR panel, 7 direct
+ * (small-K) contraction. */
+int tn_prof_enable(int on);
+int tn_prof_collect(int tag, double* total_ms, int64_t* launches, double* flops, double* bytes,
+                    int reset);
 
 #ifdef __cplusplus
 }

@@ -77,9 +77,14 @@ def lib():
         "tn_sumsq_workspace": (sz, [i64]),
         "tn_sumsq": (i32, [vp, i32, i64, vp, vp, sz, vp]),
         "tn_trace": (i32, [vp, mp, i64, i64, vp, i32, vp]),
+        "tn_normalize_workspace": (sz, [i64]),
+        "tn_normalize": (i32, [vp, i32, i64, vp, vp, sz, vp]),
         "tn_contract": (i32, [vp, mp, mp, vp, mp, mp, vp, mp, mp, i32, i32, dbl, dbl, vp]),
         "tn_qr_workspace": (sz, [i64, i64, i32]),
         "tn_qr": (i32, [vp, mp, mp, i64, i64, vp, vp, i32, vp, sz, vp]),
+        "tn_prof_enable": (i32, [i32]),
+        "tn_prof_collect": (i32, [i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), i32]),
         "tn_svd_workspace": (sz, [i64, i64, i32]),
         "tn_svd": (i32, [vp, mp, mp, i64, i64, vp, vp, vp, i32, dbl, i32,
                          ctypes.POINTER(ctypes.c_int), vp, sz, vp]),
@@ -103,6 +108,23 @@ def check(rc):
     if rc == TN_ERR_UNSUPPORTED:
         raise NotImplementedError(msg)
     raise RuntimeError(msg)
+
+
+PROF_TAGS = {"gemm_f64": 0, "gemm_f32": 1, "gemm_bf16x3": 2, "jac_gram": 3, "jac_eig": 4,
+             "jac_apply": 5, "qr_panel": 6, "direct": 7}
+
+
+def prof_enable(on=True):
+    check(lib().tn_prof_enable(1 if on else 0))
+
+
+def prof_collect(tag, reset=False):
+    """(total_ms, launches, flops, bytes) of the launches recorded for `tag`."""
+    ms, n = ctypes.c_double(0), ctypes.c_int64(0)
+    fl, by = ctypes.c_double(0), ctypes.c_double(0)
+    check(lib().tn_prof_collect(PROF_TAGS[tag], ctypes.byref(ms), ctypes.byref(n), ctypes.byref(fl),
+                                ctypes.byref(by), 1 if reset else 0))
+    return ms.value, n.value, fl.value, by.value
 
 
 def loaded_hip_runtimes():
@@ -184,6 +206,17 @@ def sumsq(t):
     ws = workspace(lib().tn_sumsq_workspace(n))
     check(lib().tn_sumsq(ptr(src), dtype_code(src), n, ptr(out), ptr(ws), ws.numel(), stream_ptr()))
     return float(out.item())
+
+
+def normalize_(t, norm_out=None):
+    """In place t /= ||t|| (device-side, no host sync); the norm is written to
+    the device scalar norm_out if given."""
+    assert t.is_contiguous()
+    n = t.numel()
+    ws = workspace(lib().tn_normalize_workspace(n))
+    out = ctypes.c_void_p(norm_out.data_ptr()) if norm_out is not None else None
+    check(lib().tn_normalize(ptr(t), dtype_code(t), n, out, ptr(ws), ws.numel(), stream_ptr()))
+    return t
 
 
 def scale_(t, alpha):

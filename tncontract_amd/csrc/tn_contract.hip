@@ -13,6 +13,7 @@
 //    output's memory order; used when K is small (e.g. the K = d = 2 MPO
 //    application), where the work is HBM-bound on the stores.
 #include "tn_internal.h"
+#include "tn_prof.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -500,6 +501,8 @@ int contract_launch(const void* A, const Modes& a_m, const Modes& a_k, const voi
     g.alpha = alpha;
     g.beta = beta;
     unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(g.nout, 256), 65536 * 4);
+    ProfScope prof(PROF_DIRECT, st, 2.0 * (double)g.nout * (double)K,
+                   (double)(dtype == TN_F64 ? 8 : 4) * ((double)g.nout * 2.0));
     if (dtype == TN_F64)
       hipLaunchKernelGGL((contract_direct<double, double>), dim3(blocks), dim3(256), 0, st, g);
     else
@@ -541,6 +544,10 @@ int contract_launch(const void* A, const Modes& a_m, const Modes& a_k, const voi
   if (S > 1) S = (int)ceil_div(K, g.kchunk);
   g.part = S > 1 ? splitk_ws : nullptr;
   dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)S);
+  const size_t esz_in = dtype == TN_F64 ? 8 : 4;
+  ProfScope prof(prec == P_F64 ? PROF_GEMM_F64 : (prec == P_F32 ? PROF_GEMM_F32 : PROF_GEMM_BF16X3),
+                 st, 2.0 * (double)M * (double)N * (double)K,
+                 (double)esz_in * ((double)M * K + (double)K * N + (double)M * N));
   if (prec == P_F64)
     hipLaunchKernelGGL(gemm_mfma_kernel<P_F64>, grid, dim3(NT), 0, st, g);
   else if (prec == P_F32)

@@ -187,6 +187,20 @@ __global__ void diag_embed_kernel(const double* __restrict__ s, int64_t k, T* __
   out[i] = (T)(r == c ? s[r] : 0.0);
 }
 
+
+// x /= sqrt(*sumsq) when positive; *norm_out = sqrt(*sumsq)
+template <typename T>
+__global__ void normalize_kernel(T* __restrict__ x, int64_t n, const double* __restrict__ sumsq,
+                                 double* __restrict__ norm_out) {
+  const double nrm = sqrt(*sumsq);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && norm_out) *norm_out = nrm;
+  if (!(nrm > 0.0)) return;
+  const T inv = (T)(1.0 / nrm);
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) x[i] *= inv;
+}
+
 static constexpr int kSumsqBlocks = 1024;
 
 }  // namespace tn
@@ -357,6 +371,26 @@ int tn_diag_embed(const double* s, int64_t k, void* out, int dtype, void* stream
     set_error(TN_ERR_UNSUPPORTED, "tn_diag_embed: unsupported dtype %d", dtype);
     return TN_ERR_UNSUPPORTED;
   }
+  TN_CHECK_LAUNCH();
+  return TN_OK;
+}
+
+size_t tn_normalize_workspace(int64_t n) { return tn_sumsq_workspace(n) + 256; }
+
+int tn_normalize(void* x, int dtype, int64_t n, double* norm_out, void* ws, size_t ws_bytes,
+                 void* stream) {
+  TN_REQUIRE(n >= 0 && x != nullptr, "tn_normalize: bad arguments");
+  TN_REQUIRE(ws && ws_bytes >= tn_normalize_workspace(n), "tn_normalize: workspace too small");
+  double* ss = (double*)((char*)ws + tn_sumsq_workspace(n));
+  TN_TRY(tn_sumsq(x, dtype, n, ss, ws, tn_sumsq_workspace(n), stream));
+  hipStream_t st = as_stream(stream);
+  unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 65536));
+  if (dtype == TN_F64)
+    hipLaunchKernelGGL(normalize_kernel<double>, dim3(blocks), dim3(256), 0, st, (double*)x, n, ss,
+                       norm_out);
+  else
+    hipLaunchKernelGGL(normalize_kernel<float>, dim3(blocks), dim3(256), 0, st, (float*)x, n, ss,
+                       norm_out);
   TN_CHECK_LAUNCH();
   return TN_OK;
 }

@@ -67,7 +67,10 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
 #pragma unroll
     for (int j = 0; j < W; ++j) x[i][j] = (r < m && j < w) ? A[r + (c0 + j) * ld] : 0.0;
   }
-  for (int j = 0; j < w; ++j) {
+  // fully unrolled over the panel columns so x[][] stays in registers
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    if (j >= w) break;
     const int64_t p = c0 + j;  // pivot row, owned by thread j (i = 0)
     double s[1] = {0.0};
 #pragma unroll
@@ -187,54 +190,105 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
   }
 }
 
-// W[c0:m, c] = (I - V T^T V^T) W[c0:m, c] for c in [cb, ce): one block per column.
+// Inner-block update W[c0:m, cb:ce] = (I - V T^T V^T) W[c0:m, cb:ce], split
+// over row chunks so the whole chip participates:
+//   qr_panel_dots:  Ypart[chunk][a][c] = sum_{r in chunk} V[r,a] W[r,c]
+//   qr_panel_update: y = sum_chunks Ypart, z = T^T y, W[r,c] -= sum_a V[r,a] z[a]
+// V[r,a] = 1 at r = c0+a, W[r, c0+a] below it, 0 above (implicit reflectors).
+constexpr int QCH = 256;  // rows per chunk
+constexpr int QCOL = 8;   // columns per block
+
+__device__ __forceinline__ double vref(const double* A, int64_t ld, int64_t c0, int a, int64_t r) {
+  return (r == c0 + a) ? 1.0 : (r > c0 + a ? A[r + (c0 + a) * ld] : 0.0);
+}
+
 template <int W>
-__global__ void __launch_bounds__(256) qr_apply_panel_kernel(double* __restrict__ A, int64_t m,
-                                                             int64_t ld, int64_t c0, int w,
-                                                             const double* __restrict__ Tp,
-                                                             int64_t cb) {
-  __shared__ double red[4 * W];
-  __shared__ double z[W];
-  const int64_t c = cb + blockIdx.x;
-  double y[W];
+__global__ void __launch_bounds__(256) qr_panel_dots(const double* __restrict__ A, int64_t m,
+                                                     int64_t ld, int64_t c0, int w, int64_t cb,
+                                                     int64_t ncol, double* __restrict__ Ypart) {
+  __shared__ double Vs[QCH][W];
+  __shared__ double red[4][W][QCOL];
+  const int64_t r0 = c0 + (int64_t)blockIdx.y * QCH;
+  const int64_t cbase = cb + (int64_t)blockIdx.x * QCOL;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < QCH * W; e += 256) {
+    int rr = e / W, a = e % W;
+    int64_t r = r0 + rr;
+    Vs[rr][a] = (a < w && r < m) ? vref(A, ld, c0, a, r) : 0.0;
+  }
+  __syncthreads();
+  double y[W][QCOL];
 #pragma unroll
-  for (int a = 0; a < W; ++a) y[a] = 0.0;
-  for (int64_t r = c0 + threadIdx.x; r < m; r += 256) {
-    double v = A[r + c * ld];
+  for (int a = 0; a < W; ++a)
 #pragma unroll
-    for (int a = 0; a < W; ++a) {
-      if (a < w) {
-        double va = (r == c0 + a) ? 1.0 : (r > c0 + a ? A[r + (c0 + a) * ld] : 0.0);
-        y[a] = fma(va, v, y[a]);
-      }
+    for (int j = 0; j < QCOL; ++j) y[a][j] = 0.0;
+  const int64_t r = r0 + tid;
+  if (r < m) {
+#pragma unroll
+    for (int j = 0; j < QCOL; ++j) {
+      int64_t c = cbase + j;
+      double v = (c < cb + ncol) ? A[r + c * ld] : 0.0;
+#pragma unroll
+      for (int a = 0; a < W; ++a) y[a][j] = fma(Vs[tid][a], v, y[a][j]);
     }
   }
 #pragma unroll
-  for (int a = 0; a < W; ++a) y[a] = wave_sum(y[a]);
-  if ((threadIdx.x & 63) == 0)
+  for (int a = 0; a < W; ++a)
 #pragma unroll
-    for (int a = 0; a < W; ++a) red[(threadIdx.x >> 6) * W + a] = y[a];
-  __syncthreads();
-  if (threadIdx.x < w) {
-    // z = T^T y, T upper triangular
-    int b = threadIdx.x;
-    double acc = 0.0;
-    for (int a = 0; a <= b; ++a) {
-      double ya = red[a] + red[W + a] + red[2 * W + a] + red[3 * W + a];
-      acc += Tp[a * w + b] * ya;
-    }
-    z[b] = acc;
+    for (int j = 0; j < QCOL; ++j) y[a][j] = wave_sum(y[a][j]);
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int a = 0; a < W; ++a)
+#pragma unroll
+      for (int j = 0; j < QCOL; ++j) red[tid >> 6][a][j] = y[a][j];
   }
   __syncthreads();
-  for (int64_t r = c0 + threadIdx.x; r < m; r += 256) {
+  if (tid < W * QCOL) {
+    int a = tid / QCOL, j = tid % QCOL;
+    double acc = red[0][a][j] + red[1][a][j] + red[2][a][j] + red[3][a][j];
+    int64_t c = cbase + j;
+    if (c < cb + ncol)
+      Ypart[((int64_t)blockIdx.y * W + a) * ncol + (c - cb)] = acc;
+  }
+}
+
+template <int W>
+__global__ void __launch_bounds__(256) qr_panel_update(double* __restrict__ A, int64_t m,
+                                                       int64_t ld, int64_t c0, int w, int64_t cb,
+                                                       int64_t ncol, int nchunk,
+                                                       const double* __restrict__ Ypart,
+                                                       const double* __restrict__ Tp) {
+  __shared__ double z[W][QCOL];
+  const int64_t r0 = c0 + (int64_t)blockIdx.y * QCH;
+  const int64_t cbase = cb + (int64_t)blockIdx.x * QCOL;
+  const int tid = threadIdx.x;
+  if (tid < W * QCOL) {
+    // z[b][j] = sum_{a <= b} T[a][b] y[a][j]
+    int b = tid / QCOL, j = tid % QCOL;
+    int64_t c = cbase + j;
     double acc = 0.0;
-#pragma unroll
-    for (int a = 0; a < W; ++a) {
-      if (a < w) {
-        double va = (r == c0 + a) ? 1.0 : (r > c0 + a ? A[r + (c0 + a) * ld] : 0.0);
-        acc = fma(va, z[a], acc);
+    if (b < w && c < cb + ncol) {
+      for (int a = 0; a <= b; ++a) {
+        double ya = 0.0;
+        for (int ch = 0; ch < nchunk; ++ch) ya += Ypart[((int64_t)ch * W + a) * ncol + (c - cb)];
+        acc += Tp[a * w + b] * ya;
       }
     }
+    z[b][j] = acc;
+  }
+  __syncthreads();
+  const int64_t r = r0 + tid;
+  if (r >= m) return;
+  double v[W];
+#pragma unroll
+  for (int a = 0; a < W; ++a) v[a] = a < w ? vref(A, ld, c0, a, r) : 0.0;
+#pragma unroll
+  for (int j = 0; j < QCOL; ++j) {
+    int64_t c = cbase + j;
+    if (c >= cb + ncol) break;
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < W; ++a) acc = fma(v[a], z[a][j], acc);
     A[r + c * ld] -= acc;
   }
 }
@@ -254,20 +308,34 @@ __global__ void __launch_bounds__(256) qr_larft_kernel(const double* __restrict_
                                                        const double* __restrict__ tau,
                                                        double* __restrict__ T) {
   __shared__ double Ts[QR_NB * QR_NB];
-  for (int i = threadIdx.x; i < nb * nb; i += 256) Ts[i] = 0.0;
+  __shared__ double Gs[QR_NB * QR_NB];
+  __shared__ double ts[QR_NB];
+  for (int i = threadIdx.x; i < nb * nb; i += 256) {
+    Ts[i] = 0.0;
+    Gs[i] = G[i];
+  }
+  if (threadIdx.x < nb) ts[threadIdx.x] = tau[threadIdx.x];
   __syncthreads();
   for (int b = 0; b < nb; ++b) {
-    const double tb = tau[b];
+    const double tb = ts[b];
     if (threadIdx.x < b) {
       int a = threadIdx.x;
       double acc = 0.0;
-      for (int c = a; c < b; ++c) acc += Ts[a * nb + c] * G[c * nb + b];
+      for (int c = a; c < b; ++c) acc += Ts[a * nb + c] * Gs[c * nb + b];
       Ts[a * nb + b] = -tb * acc;
     }
     if (threadIdx.x == 0) Ts[b * nb + b] = tb;
     __syncthreads();
   }
   for (int i = threadIdx.x; i < nb * nb; i += 256) T[i] = Ts[i];
+}
+
+__global__ void qr_init_from_kernel(const double* __restrict__ X, double* __restrict__ Q, int64_t m,
+                                    int64_t k) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m * k) return;
+  int64_t c = i / m, r = i - c * m;
+  Q[i] = r < k ? X[c * k + r] : 0.0;
 }
 
 __global__ void qr_init_q_kernel(double* __restrict__ Q, int64_t m, int64_t k) {
@@ -316,12 +384,7 @@ __global__ void gather_colmajor_kernel(const T* __restrict__ src, Modes rows, Mo
   }
 }
 
-struct QrWs {
-  double *W, *tau, *V, *Tall, *G, *Y, *Y2, *Tp, *Qw, *part;
-  size_t part_bytes;
-};
-
-static size_t qr_layout(int64_t m, int64_t n, char* base, size_t cap, QrWs* ws) {
+size_t qr_layout(int64_t m, int64_t n, char* base, size_t cap, QrWs* ws) {
   const int64_t k = std::min(m, n);
   const int64_t nblk = ceil_div(std::max<int64_t>(k, 1), QR_NB);
   const int64_t wn = std::max(n, k);
@@ -335,6 +398,7 @@ static size_t qr_layout(int64_t m, int64_t n, char* base, size_t cap, QrWs* ws) 
   w.Y = a.take<double>((size_t)(QR_NB * wn));
   w.Y2 = a.take<double>((size_t)(QR_NB * wn));
   w.Tp = a.take<double>(64);
+  w.Ypart = a.take<double>((size_t)ceil_div(m, QCH) * 8 * QR_NB);
   w.Qw = a.take<double>((size_t)(m * k));
   w.part_bytes = (size_t)16 * QR_NB * wn * sizeof(double);
   w.part = a.take<double>((size_t)16 * QR_NB * wn);
@@ -388,18 +452,24 @@ static int run_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, double
 }
 
 static int run_apply_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, const double* Tp,
-                           int64_t cb, int64_t ce, hipStream_t st) {
+                           int64_t cb, int64_t ce, double* Ypart, hipStream_t st) {
   if (ce <= cb) return TN_OK;
-  unsigned blocks = (unsigned)(ce - cb);
-  if (w > 4)
-    hipLaunchKernelGGL(qr_apply_panel_kernel<8>, dim3(blocks), dim3(256), 0, st, A, m, ld, c0, w,
-                       Tp, cb);
-  else if (w > 2)
-    hipLaunchKernelGGL(qr_apply_panel_kernel<4>, dim3(blocks), dim3(256), 0, st, A, m, ld, c0, w,
-                       Tp, cb);
-  else
-    hipLaunchKernelGGL(qr_apply_panel_kernel<2>, dim3(blocks), dim3(256), 0, st, A, m, ld, c0, w,
-                       Tp, cb);
+  const int64_t ncol = ce - cb;
+  const int nchunk = (int)ceil_div(m - c0, QCH);
+  dim3 grid((unsigned)ceil_div(ncol, QCOL), (unsigned)nchunk);
+  if (w > 4) {
+    hipLaunchKernelGGL(qr_panel_dots<8>, grid, dim3(256), 0, st, A, m, ld, c0, w, cb, ncol, Ypart);
+    hipLaunchKernelGGL(qr_panel_update<8>, grid, dim3(256), 0, st, A, m, ld, c0, w, cb, ncol,
+                       nchunk, Ypart, Tp);
+  } else if (w > 2) {
+    hipLaunchKernelGGL(qr_panel_dots<4>, grid, dim3(256), 0, st, A, m, ld, c0, w, cb, ncol, Ypart);
+    hipLaunchKernelGGL(qr_panel_update<4>, grid, dim3(256), 0, st, A, m, ld, c0, w, cb, ncol,
+                       nchunk, Ypart, Tp);
+  } else {
+    hipLaunchKernelGGL(qr_panel_dots<2>, grid, dim3(256), 0, st, A, m, ld, c0, w, cb, ncol, Ypart);
+    hipLaunchKernelGGL(qr_panel_update<2>, grid, dim3(256), 0, st, A, m, ld, c0, w, cb, ncol,
+                       nchunk, Ypart, Tp);
+  }
   TN_CHECK_LAUNCH();
   return TN_OK;
 }
@@ -414,7 +484,7 @@ int qr_factor(double* W, int64_t m, int64_t n, const QrWs& ws, hipStream_t st) {
       int wmax = panel_width(m - jj);
       int w = (int)std::min<int64_t>(wmax, j0 + nbo - jj);
       TN_TRY(run_panel(W, m, m, jj, w, ws.tau, ws.Tp, st));
-      TN_TRY(run_apply_panel(W, m, m, jj, w, ws.Tp, jj + w, j0 + nbo, st));
+      TN_TRY(run_apply_panel(W, m, m, jj, w, ws.Tp, jj + w, j0 + nbo, ws.Ypart, st));
       jj += w;
     }
     // compact-WY for the whole outer block (also used to form Q)
@@ -449,11 +519,17 @@ int qr_factor(double* W, int64_t m, int64_t n, const QrWs& ws, hipStream_t st) {
   return TN_OK;
 }
 
-// Qw (m x k column-major) = H_1 ... H_k [I; 0]
-int qr_form_q(const double* W, int64_t m, int64_t n, const QrWs& ws, double* Qw, hipStream_t st) {
+// Qw (m x k column-major) = H_1 ... H_k [X0; 0]; X0 = I when init == nullptr,
+// else the k x k column-major matrix init (ld = k).
+int qr_form_q(const double* W, int64_t m, int64_t n, const QrWs& ws, double* Qw, hipStream_t st,
+              const double* init) {
   const int64_t k = std::min(m, n);
-  hipLaunchKernelGGL(qr_init_q_kernel, dim3((unsigned)ceil_div(m * k, 256)), dim3(256), 0, st, Qw,
-                     m, k);
+  if (init == nullptr)
+    hipLaunchKernelGGL(qr_init_q_kernel, dim3((unsigned)ceil_div(m * k, 256)), dim3(256), 0, st,
+                       Qw, m, k);
+  else
+    hipLaunchKernelGGL(qr_init_from_kernel, dim3((unsigned)ceil_div(m * k, 256)), dim3(256), 0,
+                       st, init, Qw, m, k);
   TN_CHECK_LAUNCH();
   const int64_t nblk = ceil_div(k, QR_NB);
   for (int64_t b = nblk - 1; b >= 0; --b) {
@@ -513,7 +589,7 @@ extern "C" int tn_qr(const void* A, const tn_modes* a_rows, const tn_modes* a_co
                        (const float*)A, rows, cols, m, n, w.W);
   TN_CHECK_LAUNCH();
   TN_TRY(qr_factor(w.W, m, n, w, st));
-  TN_TRY(qr_form_q(w.W, m, n, w, w.Qw, st));
+  TN_TRY(qr_form_q(w.W, m, n, w, w.Qw, st, nullptr));
   dim3 tg((unsigned)ceil_div(m, 32), (unsigned)ceil_div(k, 32));
   unsigned rb = (unsigned)ceil_div(k * n, 256);
   if (dtype == TN_F64) {

@@ -19,6 +19,7 @@
 // off-diagonal of a whole sweep is <= tol.  Singular values are the vector
 // norms; they are sorted descending on the device.
 #include "tn_internal.h"
+#include "tn_prof.h"
 
 #include <cmath>
 #include <cstring>
@@ -107,33 +108,57 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
   return (unsigned long long)__double_as_longlong(x);
 }
 
+// Partner of index x in round r of the 32-position circle schedule.
+__device__ __forceinline__ int rr_partner(int x, int r) {
+  const int n1 = J2 - 1;
+  int pos = x == 0 ? 0 : 1 + ((x - 1 - r) % n1 + n1) % n1;
+  int ppos = n1 - pos;
+  return ppos == 0 ? 0 : 1 + (ppos - 1 + r) % n1;
+}
+
+// Rotation coefficients for index x against partner px: G' = J^T G J with
+// J_ll = J_hh = c, J_lh = s, J_hl = -s (l < h); row x of J^T G is
+// a*G[x] + b*G[px] with a = c and b = -s (x = l) or +s (x = h).
+__device__ __forceinline__ void rot_coef(const double (*G)[J2 + 1], int x, int px, double tol,
+                                         double& a, double& b, bool& rot) {
+  int l = min(x, px), h = max(x, px);
+  double gpq = G[l][h], gpp = G[l][l], gqq = G[h][h];
+  double c = 1.0, sn = 0.0;
+  rot = false;
+  if (gpq != 0.0 && fabs(gpq) > tol * sqrt(fmax(gpp * gqq, 0.0))) {
+    double th = (gqq - gpp) / (2.0 * gpq);
+    double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
+    c = 1.0 / sqrt(1.0 + t * t);
+    sn = t * c;
+    rot = true;
+  }
+  a = c;
+  b = x == l ? -sn : sn;
+}
+
 __global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart, int nchunk,
                                                int64_t nv, int s, int nblk, double tol,
                                                double* __restrict__ Qall, int* __restrict__ skip,
                                                unsigned long long* __restrict__ offmax) {
-  __shared__ double G[J2][J2 + 1];
-  __shared__ double Q[J2][J2 + 1];
-  __shared__ double cs[J2 / 2][2];
-  __shared__ int pp[J2 / 2][2];
+  __shared__ double G[2][J2][J2 + 1];
+  __shared__ double Q[2][J2][J2 + 1];
   __shared__ double red[4];
-  __shared__ int rotated;
+  __shared__ int rotated[3];
   const int pi = blockIdx.x, tid = threadIdx.x;
-  int I, J;
-  pair_blocks(pi, s, nblk, I, J);
   for (int e = tid; e < J2 * J2; e += 256) {
     double acc = 0.0;
     for (int c = 0; c < nchunk; ++c) acc += Gpart[((int64_t)pi * nchunk + c) * (J2 * J2) + e];
-    G[e / J2][e % J2] = acc;
-    Q[e / J2][e % J2] = (e / J2 == e % J2) ? 1.0 : 0.0;
+    G[0][e / J2][e % J2] = acc;
+    Q[0][e / J2][e % J2] = (e / J2 == e % J2) ? 1.0 : 0.0;
   }
   __syncthreads();
-  // off-diagonal measure (all pairs p < q of the 32 vectors)
+  // off-diagonal measure over all pairs p < q of the 32 vectors
   double off = 0.0;
   for (int e = tid; e < J2 * J2; e += 256) {
     int p = e / J2, q = e % J2;
     if (p < q) {
-      double d = G[p][p] * G[q][q];
-      if (d > 0.0) off = fmax(off, fabs(G[p][q]) / sqrt(d));
+      double d = G[0][p][p] * G[0][q][q];
+      if (d > 0.0) off = fmax(off, fabs(G[0][p][q]) / sqrt(d));
     }
   }
   off = wave_max(off);
@@ -146,59 +171,58 @@ __global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart,
     return;
   }
   if (tid == 0) skip[pi] = 0;
-  // parallel cyclic Jacobi on the 32 x 32 Gram matrix
-  const double inner_tol = 1e-15;
-  for (int sweep = 0; sweep < 30; ++sweep) {
-    if (tid == 0) rotated = 0;
-    __syncthreads();
+  // Parallel cyclic Jacobi, one barrier per round.  Thread t owns the 2x2
+  // block (row pair t/16, column pair t%16) of G' = J^T G J and of Q' = Q J,
+  // so each thread evaluates only the two rotations its block needs.
+  const double itol = tol;
+  int cur = 0;
+  // sweep flags rotate over 3 slots: slot (s+1)%3 is cleared during sweep s,
+  // so no thread can still be reading it and no thread writes it yet.
+  if (tid == 0) rotated[0] = rotated[1] = rotated[2] = 0;
+  __syncthreads();
+  const int rp = tid >> 4, cp = tid & 15;
+  for (int sweep = 0; sweep < 8; ++sweep) {
+    const int slot = sweep % 3;
+    if (tid == 0) rotated[(sweep + 1) % 3] = 0;
     for (int r = 0; r < J2 - 1; ++r) {
-      if (tid < J2 / 2) {
-        int p = tid == 0 ? 0 : 1 + (tid - 1 + r) % (J2 - 1);
-        int q = 1 + (J2 - 2 - tid + r) % (J2 - 1);
-        if (p > q) { int t = p; p = q; q = t; }
-        double gpq = G[p][q], gpp = G[p][p], gqq = G[q][q];
-        double c = 1.0, sn = 0.0;
-        if (fabs(gpq) > inner_tol * sqrt(fmax(gpp * gqq, 0.0)) && gpq != 0.0) {
-          double th = (gqq - gpp) / (2.0 * gpq);
-          double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
-          c = 1.0 / sqrt(1.0 + t * t);
-          sn = t * c;
-          rotated = 1;
-        }
-        cs[tid][0] = c; cs[tid][1] = sn;
-        pp[tid][0] = p; pp[tid][1] = q;
-      }
-      __syncthreads();
-      // rows: G <- J^T G
-      for (int e = tid; e < (J2 / 2) * J2; e += 256) {
-        int k = e / J2, col = e % J2;
-        int p = pp[k][0], q = pp[k][1];
-        double c = cs[k][0], sn = cs[k][1];
-        double gp = G[p][col], gq = G[q][col];
-        G[p][col] = c * gp - sn * gq;
-        G[q][col] = sn * gp + c * gq;
-      }
-      __syncthreads();
-      // columns: G <- G J, Q <- Q J
-      for (int e = tid; e < (J2 / 2) * J2; e += 256) {
-        int k = e / J2, row = e % J2;
-        int p = pp[k][0], q = pp[k][1];
-        double c = cs[k][0], sn = cs[k][1];
-        double gp = G[row][p], gq = G[row][q];
-        G[row][p] = c * gp - sn * gq;
-        G[row][q] = sn * gp + c * gq;
-        double qp = Q[row][p], qq = Q[row][q];
-        Q[row][p] = c * qp - sn * qq;
-        Q[row][q] = sn * qp + c * qq;
-      }
+      const int nxt = cur ^ 1;
+      // pair p occupies circle positions (p, 31 - p)
+      int x0 = rp == 0 ? 0 : 1 + (rp - 1 + r) % (J2 - 1);
+      int x1 = 1 + (J2 - 2 - rp + r) % (J2 - 1);
+      int y0 = cp == 0 ? 0 : 1 + (cp - 1 + r) % (J2 - 1);
+      int y1 = 1 + (J2 - 2 - cp + r) % (J2 - 1);
+      double ax0, bx0, ax1, bx1, ay0, by0, ay1, by1;
+      bool rx, ry;
+      rot_coef(G[cur], x0, x1, itol, ax0, bx0, rx);
+      rot_coef(G[cur], x1, x0, itol, ax1, bx1, rx);
+      rot_coef(G[cur], y0, y1, itol, ay0, by0, ry);
+      rot_coef(G[cur], y1, y0, itol, ay1, by1, ry);
+      const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
+      const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
+      // rows first: R = J^T G restricted to rows {x0,x1}
+      const double r00 = ax0 * g00 + bx0 * g10, r01 = ax0 * g01 + bx0 * g11;
+      const double r10 = ax1 * g10 + bx1 * g00, r11 = ax1 * g11 + bx1 * g01;
+      // then columns: G' = R J restricted to columns {y0,y1}
+      G[nxt][x0][y0] = ay0 * r00 + by0 * r01;
+      G[nxt][x0][y1] = ay1 * r01 + by1 * r00;
+      G[nxt][x1][y0] = ay0 * r10 + by0 * r11;
+      G[nxt][x1][y1] = ay1 * r11 + by1 * r10;
+      // Q' = Q J: rows {x0,x1} (any row set), columns {y0,y1}
+      const double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
+      const double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
+      Q[nxt][x0][y0] = ay0 * q00 + by0 * q01;
+      Q[nxt][x0][y1] = ay1 * q01 + by1 * q00;
+      Q[nxt][x1][y0] = ay0 * q10 + by0 * q11;
+      Q[nxt][x1][y1] = ay1 * q11 + by1 * q10;
+      if (rp == cp && rx) rotated[slot] = 1;
+      cur = nxt;
       __syncthreads();
     }
-    const int any = rotated;
-    __syncthreads();  // everyone has read `rotated` before thread 0 resets it
-    if (!any) break;
+    const int again = rotated[slot];
+    if (!again) break;
   }
   double* out = Qall + (int64_t)pi * (J2 * J2);
-  for (int e = tid; e < J2 * J2; e += 256) out[e] = Q[e / J2][e % J2];
+  for (int e = tid; e < J2 * J2; e += 256) out[e] = Q[cur][e / J2][e % J2];
 }
 
 // rows [ch*JCH, ...) of the stacked [Wv (L rows); Z (nv rows)] for pair pi
@@ -270,41 +294,39 @@ __global__ void __launch_bounds__(1024) rank_sort(const double* __restrict__ sig
   }
 }
 
-// U (m x k row-major) and Vt (k x n row-major) from Wv / Z.
+// U (m x k row-major) and Vt (k x n row-major).  Column j of U is vector
+// perm[j] of Uvec (ld ldu), row j of Vt is vector perm[j] of Vvec (ld ldv);
+// a "scaled" source holds sigma_j * (unit vector) and is divided by S[j].
 template <typename T>
-__global__ void svd_out_vec(const double* __restrict__ Wv, int64_t L, const double* __restrict__ Z,
-                            int64_t k, const int* __restrict__ perm, const double* __restrict__ S,
-                            int tall, T* __restrict__ U, T* __restrict__ Vt, int64_t m,
-                            int64_t n) {
-  // tall (m >= n): U[i, j] = Wv[perm j][i] / S[j];  Vt[j, c] = Z[perm j][c]
-  // wide (m <  n): U[i, j] = Z[perm j][i];           Vt[j, c] = Wv[perm j][c] / S[j]
+__global__ void svd_out_kernel(const double* __restrict__ Uvec, int64_t ldu, int u_scaled,
+                               const double* __restrict__ Vvec, int64_t ldv, int v_scaled,
+                               int64_t k, const int* __restrict__ perm,
+                               const double* __restrict__ S, T* __restrict__ U,
+                               T* __restrict__ Vt, int64_t m, int64_t n) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t total = m * k + k * n;
-  if (idx >= total) return;
+  if (idx >= m * k + k * n) return;
   if (idx < m * k) {
     int64_t i = idx / k, j = idx - i * k;
-    int64_t src = perm[j];
-    double v;
-    if (tall) {
-      double s = S[j];
-      v = s > 0.0 ? Wv[src * L + i] / s : 0.0;
-    } else {
-      v = Z[src * k + i];
-    }
+    double v = Uvec[(int64_t)perm[j] * ldu + i];
+    if (u_scaled) v = S[j] > 0.0 ? v / S[j] : 0.0;
     U[idx] = (T)v;
   } else {
     int64_t e = idx - m * k;
     int64_t j = e / n, c = e - j * n;
-    int64_t src = perm[j];
-    double v;
-    if (tall) {
-      v = Z[src * k + c];
-    } else {
-      double s = S[j];
-      v = s > 0.0 ? Wv[src * L + c] / s : 0.0;
-    }
+    double v = Vvec[(int64_t)perm[j] * ldv + c];
+    if (v_scaled) v = S[j] > 0.0 ? v / S[j] : 0.0;
     Vt[e] = (T)v;
   }
+}
+
+// Rows of the upper-triangular R (in the column-major QR workspace, ld = L)
+// as contiguous vectors: V[j*k + c] = R[j][c].
+__global__ void r_rows_kernel(const double* __restrict__ W, int64_t L, int64_t k,
+                              double* __restrict__ V) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= k * k) return;
+  int64_t j = idx / k, c = idx - j * k;
+  V[idx] = c >= j ? W[j + c * L] : 0.0;
 }
 
 template <typename T>
@@ -327,33 +349,92 @@ __global__ void init_identity(double* __restrict__ Z, int64_t k) {
 }
 
 struct SvdWs {
-  double *Wv, *Z, *Gpart, *Q, *sig, *S;
+  double *Wv, *Z, *Gpart, *Q, *sig, *S, *R, *QZ;
   int *skip, *perm;
   unsigned long long* off;
-  int nblk, npairs, nchW, nchZ, nch;
+  QrWs qr;
+  int precond;
 };
+
+// Precondition with a QR when the vectors are much longer than their count.
+static bool use_precond(int64_t L, int64_t k) { return k >= 32 && L >= 2 * k; }
 
 static size_t svd_layout(int64_t m, int64_t n, char* base, size_t cap, SvdWs* out) {
   const int64_t k = std::min(m, n), L = std::max(m, n);
   SvdWs w{};
-  w.nblk = (int)ceil_div(std::max<int64_t>(k, 1), JB);
-  if (w.nblk % 2) w.nblk += 1;
-  w.npairs = w.nblk / 2;
-  w.nchW = (int)ceil_div(L, JCH);
-  w.nchZ = (int)ceil_div(k, JCH);
-  w.nch = w.nchW;
+  w.precond = use_precond(L, k) ? 1 : 0;
+  const int64_t Lj = w.precond ? k : L;  // vector length inside the Jacobi sweeps
+  const int nblk = (int)ceil_div(std::max<int64_t>(k, 1), JB) + ((ceil_div(std::max<int64_t>(k, 1), JB)) % 2);
+  const int npairs = nblk / 2;
   Arena a{base, cap, 0};
   w.Wv = a.take<double>((size_t)(L * k));
   w.Z = a.take<double>((size_t)(k * k));
-  w.Gpart = a.take<double>((size_t)w.npairs * w.nchW * J2 * J2);
-  w.Q = a.take<double>((size_t)w.npairs * J2 * J2);
+  w.Gpart = a.take<double>((size_t)npairs * ceil_div(Lj, JCH) * J2 * J2);
+  w.Q = a.take<double>((size_t)npairs * J2 * J2);
   w.sig = a.take<double>((size_t)k);
   w.S = a.take<double>((size_t)k);
-  w.skip = a.take<int>((size_t)w.npairs);
+  w.skip = a.take<int>((size_t)npairs);
   w.perm = a.take<int>((size_t)k);
   w.off = a.take<unsigned long long>(4);
+  if (w.precond) {
+    w.R = a.take<double>((size_t)(k * k));
+    w.QZ = a.take<double>((size_t)(L * k));
+    size_t qbytes = qr_layout(L, k, nullptr, 0, nullptr) + 256;
+    char* qbase = a.take<char>(qbytes);
+    if (base) qr_layout(L, k, qbase, qbytes, &w.qr);
+  }
   if (out) *out = w;
   return a.used;
+}
+
+// One-sided block Jacobi on k vectors of length L (Wv, vector-contiguous),
+// accumulating the rotations into Z (k x k, vector-contiguous).
+static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& w, double tol,
+                      int max_sweeps, int* sweeps_out, bool* converged, hipStream_t st) {
+  int nblk = (int)ceil_div(std::max<int64_t>(k, 1), JB);
+  if (nblk % 2) nblk += 1;
+  const int npairs = nblk / 2;
+  const int nchW = (int)ceil_div(L, JCH), nchZ = (int)ceil_div(k, JCH);
+  hipLaunchKernelGGL(init_identity, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, Z, k);
+  TN_CHECK_LAUNCH();
+  int sweep = 0;
+  *converged = false;
+  unsigned long long host_off = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    TN_HIP(hipMemsetAsync(w.off, 0, sizeof(unsigned long long), st));
+    // algorithmic work per step: Gram 2 L (2b)^2 per pair, apply 2 (L + k) (2b)^2 per pair
+    const double gram_fl = 2.0 * (double)L * J2 * J2 * npairs;
+    const double apply_fl = 2.0 * (double)(L + k) * J2 * J2 * npairs;
+    for (int s = 0; s < nblk - 1; ++s) {
+      {
+        ProfScope p(PROF_JAC_GRAM, st, gram_fl, 8.0 * (double)L * J2 * npairs);
+        hipLaunchKernelGGL(jac_gram, dim3(npairs, nchW), dim3(256), 0, st, Wv, L, k, s, nblk,
+                           nchW, w.Gpart);
+      }
+      {
+        ProfScope p(PROF_JAC_EIG, st, 0.0, 8.0 * (double)nchW * J2 * J2 * npairs);
+        hipLaunchKernelGGL(jac_eig, dim3(npairs), dim3(256), 0, st, w.Gpart, nchW, k, s, nblk,
+                           tol, w.Q, w.skip, w.off);
+      }
+      {
+        ProfScope p(PROF_JAC_APPLY, st, apply_fl, 16.0 * (double)(L + k) * J2 * npairs);
+        hipLaunchKernelGGL(jac_apply, dim3(npairs, nchW + nchZ), dim3(256), 0, st, Wv, L, Z, k,
+                           s, nblk, nchW, w.Q, w.skip);
+      }
+      TN_CHECK_LAUNCH();
+    }
+    TN_HIP(hipMemcpyAsync(&host_off, w.off, sizeof(host_off), hipMemcpyDeviceToHost, st));
+    TN_HIP(hipStreamSynchronize(st));
+    double off;
+    memcpy(&off, &host_off, sizeof(off));
+    if (off <= tol) {
+      *converged = true;
+      ++sweep;
+      break;
+    }
+  }
+  if (sweeps_out) *sweeps_out = sweep;
+  return TN_OK;
 }
 
 }  // namespace tn
@@ -379,12 +460,12 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
   TN_REQUIRE(A && U && S && Vt, "tn_svd: null pointer");
   TN_REQUIRE(ws && ws_bytes >= tn_svd_workspace(m, n, dtype), "tn_svd: workspace too small");
   TN_REQUIRE(k <= (int64_t)1 << 30, "tn_svd: too large");
-  if (tol <= 0.0) tol = std::max(1e-15, std::sqrt((double)L) * 2.220446049250313e-16);
   if (max_sweeps <= 0) max_sweeps = 40;
   hipStream_t st = as_stream(stream);
   SvdWs w{};
   svd_layout(m, n, (char*)ws, ws_bytes, &w);
   const int tall = m >= n ? 1 : 0;
+  // X (L x k, column-major) holds the vectors: columns of A (tall) or rows (wide)
   unsigned gb = (unsigned)std::min<int64_t>(ceil_div(m * n, 256), 65536);
   if (dtype == TN_F64)
     hipLaunchKernelGGL(gather_vectors<double>, dim3(gb), dim3(256), 0, st, (const double*)A, rows,
@@ -393,43 +474,47 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
     hipLaunchKernelGGL(gather_vectors<float>, dim3(gb), dim3(256), 0, st, (const float*)A, rows,
                        cols, m, n, tall, w.Wv);
   TN_CHECK_LAUNCH();
-  hipLaunchKernelGGL(init_identity, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, w.Z, k);
-  TN_CHECK_LAUNCH();
-  int sweep = 0;
   bool converged = false;
-  unsigned long long host_off = 0;
-  for (; sweep < max_sweeps; ++sweep) {
-    TN_HIP(hipMemsetAsync(w.off, 0, sizeof(unsigned long long), st));
-    for (int s = 0; s < w.nblk - 1; ++s) {
-      hipLaunchKernelGGL(jac_gram, dim3(w.npairs, w.nchW), dim3(256), 0, st, w.Wv, L, k, s,
-                         w.nblk, w.nchW, w.Gpart);
-      hipLaunchKernelGGL(jac_eig, dim3(w.npairs), dim3(256), 0, st, w.Gpart, w.nchW, k, s, w.nblk,
-                         tol, w.Q, w.skip, w.off);
-      hipLaunchKernelGGL(jac_apply, dim3(w.npairs, w.nchW + w.nchZ), dim3(256), 0, st, w.Wv, L,
-                         w.Z, k, s, w.nblk, w.nchW, w.Q, w.skip);
-      TN_CHECK_LAUNCH();
-    }
-    TN_HIP(hipMemcpyAsync(&host_off, w.off, sizeof(host_off), hipMemcpyDeviceToHost, st));
-    TN_HIP(hipStreamSynchronize(st));
-    double off;
-    memcpy(&off, &host_off, sizeof(off));
-    if (off <= tol) {
-      converged = true;
-      ++sweep;
-      break;
-    }
+  const double* Uvec;
+  const double* Vvec;
+  int64_t ldu, ldv;
+  int u_scaled, v_scaled;
+  const double* orth_src;  // vectors whose norms are the singular values
+  int64_t orth_len;
+  if (w.precond) {
+    // X = Q R; one-sided Jacobi on the rows of R (R^T Z = W''), then
+    // X = (Q Z) Sigma (W''/Sigma)^T.
+    TN_TRY(qr_factor(w.Wv, L, k, w.qr, st));
+    hipLaunchKernelGGL(r_rows_kernel, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, w.Wv,
+                       L, k, w.R);
+    TN_CHECK_LAUNCH();
+    if (tol <= 0.0) tol = std::max(1e-15, (double)k * 2.220446049250313e-16);
+    TN_TRY(jacobi_run(w.R, k, w.Z, k, w, tol, max_sweeps, sweeps_out, &converged, st));
+    TN_TRY(qr_form_q(w.Wv, L, k, w.qr, w.QZ, st, w.Z));
+    orth_src = w.R;
+    orth_len = k;
+    // left vectors of X: Q Z (orthonormal); right vectors of X: rows of R rotated
+    if (tall) { Uvec = w.QZ; ldu = L; u_scaled = 0; Vvec = w.R; ldv = k; v_scaled = 1; }
+    else { Uvec = w.R; ldu = k; u_scaled = 1; Vvec = w.QZ; ldv = L; v_scaled = 0; }
+  } else {
+    // LAPACK dgesvj convergence threshold: (vector length) * eps
+    if (tol <= 0.0) tol = std::max(1e-15, (double)L * 2.220446049250313e-16);
+    TN_TRY(jacobi_run(w.Wv, L, w.Z, k, w, tol, max_sweeps, sweeps_out, &converged, st));
+    orth_src = w.Wv;
+    orth_len = L;
+    if (tall) { Uvec = w.Wv; ldu = L; u_scaled = 1; Vvec = w.Z; ldv = k; v_scaled = 0; }
+    else { Uvec = w.Z; ldu = k; u_scaled = 0; Vvec = w.Wv; ldv = L; v_scaled = 1; }
   }
-  if (sweeps_out) *sweeps_out = sweep;
-  hipLaunchKernelGGL(vec_norms, dim3((unsigned)k), dim3(256), 0, st, w.Wv, L, w.sig);
+  hipLaunchKernelGGL(vec_norms, dim3((unsigned)k), dim3(256), 0, st, orth_src, orth_len, w.sig);
   hipLaunchKernelGGL(rank_sort, dim3(1), dim3(1024), 0, st, w.sig, k, w.perm, S);
   TN_CHECK_LAUNCH();
   unsigned ob = (unsigned)ceil_div(m * k + k * n, 256);
   if (dtype == TN_F64)
-    hipLaunchKernelGGL(svd_out_vec<double>, dim3(ob), dim3(256), 0, st, w.Wv, L, w.Z, k, w.perm, S,
-                       tall, (double*)U, (double*)Vt, m, n);
+    hipLaunchKernelGGL(svd_out_kernel<double>, dim3(ob), dim3(256), 0, st, Uvec, ldu, u_scaled,
+                       Vvec, ldv, v_scaled, k, w.perm, S, (double*)U, (double*)Vt, m, n);
   else
-    hipLaunchKernelGGL(svd_out_vec<float>, dim3(ob), dim3(256), 0, st, w.Wv, L, w.Z, k, w.perm, S,
-                       tall, (float*)U, (float*)Vt, m, n);
+    hipLaunchKernelGGL(svd_out_kernel<float>, dim3(ob), dim3(256), 0, st, Uvec, ldu, u_scaled,
+                       Vvec, ldv, v_scaled, k, w.perm, S, (float*)U, (float*)Vt, m, n);
   TN_CHECK_LAUNCH();
   if (!converged) {
     set_error(TN_ERR_NOCONV, "tn_svd: Jacobi did not converge in %d sweeps", max_sweeps);
