@@ -121,6 +121,48 @@ int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_
            void* U, double* S, void* Vt, int dtype, double tol, int max_sweeps, int* sweeps_out,
            void* ws, size_t ws_bytes, void* stream);
 
+/* ---- batched uneven contraction + truncation --------------------------
+ * No single reference call: each job is what the reference does with one
+ * tensordot (tensor.py:735) followed by truncated_svd (tensor.py:1112) with a
+ * relative threshold -- here for many independent jobs of different sizes in
+ * shared launches (grouped FP32-MFMA GEMM with autotuned tile shapes, then a
+ * batched one-sided block Jacobi with FP64 rotations).
+ * C_j = L_j R_j (m x n); C_j = U S V^T (thin, sorted descending).  kept_out[j]
+ * (host) = #{sigma/sigma_0 > rel_threshold}, capped at chi_max (>0).  The
+ * first kept columns of U / rows of Vt are the truncated factors. */
+typedef struct tn_trunc_job {
+  const void* L; /* m x k FP32 row-major */
+  const void* R; /* k x n FP32 row-major */
+  int64_t m, k, n;
+  void* U;       /* m x min(m,n) FP32 row-major */
+  double* S;     /* min(m,n), FP64, descending */
+  void* Vt;      /* min(m,n) x n FP32 row-major */
+} tn_trunc_job;
+size_t tn_batched_truncate_workspace(const tn_trunc_job* jobs, int njobs);
+int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshold, int chi_max,
+                        int max_sweeps, int* kept_out, int* sweeps_out, void* ws, size_t ws_bytes,
+                        void* stream);
+
+/* ---- learned-rank truncation (no reference counterpart: the reference
+ * always runs the full SVD at tensor.py:915) -------------------------------
+ * Rank-prediction MLP 34 -> 64 -> 64 -> 1 (ReLU, FP32).  Parameters are one
+ * FP32 vector of tn_mlp_param_count() floats: W1[34][64], b1[64],
+ * W2[64][64], b2[64], w3[64], b3.  X: B x 34 features, Y: B outputs; H1/H2
+ * (B x 64, may be NULL in inference) keep the activations for the backward
+ * pass.  tn_mlp_backward writes d/dP of (1/B) sum (y - t)^2 into grad (P
+ * floats) using per_sample (B x P floats) as deterministic scratch. */
+int tn_mlp_param_count(void);
+int tn_mlp_forward(const float* X, int B, const float* P, float* Y, float* H1, float* H2,
+                   void* stream);
+int tn_mlp_backward(const float* X, const float* T, int B, const float* P, const float* Y,
+                    const float* H1, const float* H2, float* grad, float* per_sample,
+                    void* stream);
+int tn_adam_step(float* P, const float* grad, float* m, float* v, int n, float lr, float beta1,
+                 float beta2, float eps, int step, void* stream);
+/* n standard-normal samples (counter-based, deterministic per seed): the
+ * Gaussian test matrix of the randomized range finder. */
+int tn_randn(void* out, int dtype, int64_t n, unsigned long long seed, void* stream);
+
 /* ---- launch timing (no reference counterpart: measurement support) ------
  * tn_prof_enable(1) makes the library record a HIP event pair, on the launch
  * stream, around every launch of the instrumented kernels; tn_prof_collect

@@ -1,0 +1,94 @@
+"""Batched uneven contraction + truncated SVD (many independent jobs).
+
+``batched_truncate(lefts, rights, rel_threshold, chi)`` computes, for every
+job j, C_j = L_j @ R_j and its truncated SVD (keep sigma/sigma_0 >
+rel_threshold, at most chi), all jobs in shared launches of
+libtncontract_amd (grouped FP32-MFMA GEMM with autotuned tile shapes,
+batched one-sided Jacobi with FP64 rotations).  Per job this is what the
+reference computes with ``contract`` (tensor.py:635) followed by
+``truncated_svd(..., absolute=False)`` (tensor.py:1112).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from tncontract_amd import _lib as L
+
+
+class tn_trunc_job(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_void_p), ("R", ctypes.c_void_p), ("m", ctypes.c_int64),
+                ("k", ctypes.c_int64), ("n", ctypes.c_int64), ("U", ctypes.c_void_p),
+                ("S", ctypes.c_void_p), ("Vt", ctypes.c_void_p)]
+
+
+def _bind():
+    lib = L.lib()
+    if not hasattr(lib, "_tn_batched_bound"):
+        jp = ctypes.POINTER(tn_trunc_job)
+        lib.tn_batched_truncate_workspace.restype = ctypes.c_size_t
+        lib.tn_batched_truncate_workspace.argtypes = [jp, ctypes.c_int]
+        lib.tn_batched_truncate.restype = ctypes.c_int
+        lib.tn_batched_truncate.argtypes = [jp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
+                                            ctypes.c_size_t, ctypes.c_void_p]
+        lib._tn_batched_bound = True
+    return lib
+
+
+def batched_truncate(lefts, rights, rel_threshold=1e-8, chi=0, max_sweeps=40):
+    """lefts[j]: (m_j, k_j) FP32 device tensor, rights[j]: (k_j, n_j).
+    Returns (U, S, Vt, kept, sweeps): per-job full thin factors (device) and
+    host arrays of kept ranks and Jacobi sweeps."""
+    lib = _bind()
+    nj = len(lefts)
+    jobs = (tn_trunc_job * nj)()
+    outs = []
+    keep_alive = []
+    for j, (a, b) in enumerate(zip(lefts, rights)):
+        a = L.contiguous(L.convert(a, torch.float32))
+        b = L.contiguous(L.convert(b, torch.float32))
+        m, k = a.shape
+        k2, n = b.shape
+        if k != k2:
+            raise ValueError("job %d: inner dimensions differ (%d vs %d)" % (j, k, k2))
+        r = min(m, n)
+        U = torch.empty((m, r), dtype=torch.float32, device=a.device)
+        S = torch.empty((r,), dtype=torch.float64, device=a.device)
+        Vt = torch.empty((r, n), dtype=torch.float32, device=a.device)
+        jobs[j] = tn_trunc_job(a.data_ptr(), b.data_ptr(), m, k, n, U.data_ptr(), S.data_ptr(),
+                               Vt.data_ptr())
+        outs.append((U, S, Vt))
+        keep_alive += [a, b]
+    ws = L.workspace(lib.tn_batched_truncate_workspace(jobs, nj))
+    kept = (ctypes.c_int * nj)()
+    sweeps = (ctypes.c_int * nj)()
+    L.check(lib.tn_batched_truncate(jobs, nj, float(rel_threshold), int(chi or 0), int(max_sweeps),
+                                    kept, sweeps, L.ptr(ws), ws.numel(), L.stream_ptr()))
+    return ([o[0] for o in outs], [o[1] for o in outs], [o[2] for o in outs],
+            np.array(kept[:]), np.array(sweeps[:]))
+
+
+def config3_jobs(njobs, seed, chi_lo=16, chi_hi=512, d=2):
+    """Seeded jobs of the batched-truncation workload: L_j (chi_l d x chi_m)
+    and R_j (chi_m x d chi_r) with L_j R_j = U diag(sigma) V^T, U, V random
+    orthogonal (QR of Gaussians), sigma_k = exp(-k / tau), tau ~ U[4, 64].
+    Returns host float32 arrays and the exact spectra."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(njobs):
+        cl, cm, cr = (int(x) for x in rng.integers(chi_lo, chi_hi + 1, size=3))
+        tau = float(rng.uniform(4.0, 64.0))
+        m, n = cl * d, d * cr
+        r = min(cm, m, n)
+        u, _ = np.linalg.qr(rng.standard_normal((m, r)))
+        v, _ = np.linalg.qr(rng.standard_normal((n, r)))
+        x, _ = np.linalg.qr(rng.standard_normal((cm, r)))  # cm x r, orthonormal columns
+        sig = np.exp(-np.arange(r) / tau)
+        Lm = ((u * sig) @ x.T).astype(np.float32)          # m x cm
+        Rm = (x @ v.T).astype(np.float32)                   # cm x n
+        out.append((Lm, Rm, sig))
+    return out

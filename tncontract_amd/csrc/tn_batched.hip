@@ -1,0 +1,713 @@
+// Batched uneven contraction + truncated SVD (many independent jobs of
+// different sizes, e.g. two-site MPS updates: C_j = L_j R_j, then a truncated
+// SVD of C_j).  The reference has no batched path: each job is one
+// tensordot (tensor.py:735) + one tensor_svd/truncated_svd (tensor.py:830,
+// 1112).  Everything below runs every job in shared launches:
+//
+//  * grouped GEMM (FP32 MFMA, v_mfma_f32_16x16x4): one launch covers the
+//    tiles of all jobs (block -> (job, tile) by binary search over a tile
+//    prefix); the epilogue writes each C_j as FP64 directly into the Jacobi
+//    vector layout (columns of C_j if tall, rows if wide).  Two tile shapes
+//    (64x64, 128x128); a small autotuner times both per size bucket on first
+//    use and caches the choice.
+//  * batched one-sided block Jacobi: every job advances its own round-robin
+//    schedule (step g mod (nblk_j - 1)) in the same gram / eig / apply
+//    launches; a per-job sweep-end kernel retires converged jobs on the
+//    device; the host polls the number of active jobs every few steps.
+//  * per-job sort, rank selection (sigma/sigma_0 > threshold, <= chi) and
+//    output of U, S, V^T.
+#include "tn_internal.h"
+#include "tn_prof.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace tn {
+
+typedef float f32x4b __attribute__((ext_vector_type(4)));
+typedef double f64x4b __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------- grouped GEMM
+struct GJob {
+  const float* A;  // M x K row-major
+  const float* B;  // K x N row-major
+  double* C;       // output: element (r, c) at C[r * crs + c * ccs]
+  int32_t M, N, K;
+  int64_t crs, ccs;
+};
+
+template <int FR>  // MFMA fragments per wave along each dimension (tile = 32*FR)
+__global__ void __launch_bounds__(256) gemm_grouped_f32(const GJob* __restrict__ jobs,
+                                                       const int* __restrict__ tile_base,
+                                                       const int* __restrict__ job_ids, int njobs) {
+  constexpr int BT = 32 * FR;  // block tile (square)
+  constexpr int BK = 16;
+  constexpr int LD = BT + 2;
+  __shared__ float As[BK * LD];
+  __shared__ float Bs[BK * LD];
+  // locate job: largest j with tile_base[j] <= blockIdx.x
+  int lo = 0, hi = njobs - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tile_base[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  const GJob g = jobs[job_ids[lo]];
+  const int t = b - tile_base[lo];
+  const int ntn = (g.N + BT - 1) / BT;
+  const int m0 = (t / ntn) * BT, n0 = (t % ntn) * BT;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * (16 * FR), wc = (wave & 1) * (16 * FR);
+  const int lr = lane & 15, lk = lane >> 4;
+  f32x4b acc[FR][FR];
+#pragma unroll
+  for (int i = 0; i < FR; ++i)
+#pragma unroll
+    for (int j = 0; j < FR; ++j) acc[i][j] = f32x4b{0, 0, 0, 0};
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    // A tile: lanes walk k (unit stride); B tile: lanes walk n (unit stride)
+    for (int e = tid; e < BT * BK; e += 256) {
+      int kk = e % BK, r = e / BK;
+      int m = m0 + r, k = k0 + kk;
+      As[kk * LD + r] = (m < g.M && k < g.K) ? g.A[(int64_t)m * g.K + k] : 0.f;
+      int c = e % BT, kb = e / BT;
+      int n = n0 + c, k2 = k0 + kb;
+      Bs[kb * LD + c] = (n < g.N && k2 < g.K) ? g.B[(int64_t)k2 * g.N + n] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      float af[FR], bf[FR];
+#pragma unroll
+      for (int i = 0; i < FR; ++i) af[i] = As[(ks + lk) * LD + wr + 16 * i + lr];
+#pragma unroll
+      for (int j = 0; j < FR; ++j) bf[j] = Bs[(ks + lk) * LD + wc + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < FR; ++i)
+#pragma unroll
+        for (int j = 0; j < FR; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < FR; ++i)
+#pragma unroll
+    for (int j = 0; j < FR; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int r = m0 + wr + 16 * i + 4 * lk + e, c = n0 + wc + 16 * j + lr;
+        if (r < g.M && c < g.N) g.C[(int64_t)r * g.crs + (int64_t)c * g.ccs] = (double)acc[i][j][e];
+      }
+}
+
+// ---------------------------------------------------------------- batched Jacobi
+constexpr int BJB = 16, BJ2 = 32, BCH = 256;
+
+struct BJob {
+  int64_t wv;     // offset of the job's vectors (k vectors of length L)
+  int64_t z;      // offset of the job's rotation accumulator (k x k)
+  int32_t L, k, nblk, nch, nchz;
+  int32_t gram_base, eig_base;
+  double tol;
+};
+
+__device__ __forceinline__ int brr(int pos, int step, int nblk) {
+  return pos == 0 ? 0 : 1 + (pos - 1 + step) % (nblk - 1);
+}
+
+__global__ void __launch_bounds__(256) bjac_gram(double* __restrict__ Wv, const BJob* __restrict__ jobs,
+                                                 const int* __restrict__ it_job,
+                                                 const int* __restrict__ it_pair,
+                                                 const int* __restrict__ it_chunk,
+                                                 const int* __restrict__ done, int g,
+                                                 double* __restrict__ Gpart) {
+  __shared__ double Ps[64][BJ2 + 1];
+  const int item = blockIdx.x;
+  const int j = it_job[item];
+  if (done[j]) return;
+  const BJob J = jobs[j];
+  const int s = g % (J.nblk - 1);
+  const int pi = it_pair[item], ch = it_chunk[item];
+  const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ti = wave >> 1, tj = wave & 1;
+  const int64_t r0 = (int64_t)ch * BCH, r1 = min((int64_t)J.L, r0 + BCH);
+  const double* base = Wv + J.wv;
+  f64x4b acc = {0, 0, 0, 0};
+  for (int64_t rb = r0; rb < r1; rb += 64) {
+    for (int e = tid; e < 64 * BJ2; e += 256) {
+      int row = e & 63, col = e >> 6;
+      int64_t v = col < BJB ? (int64_t)I * BJB + col : (int64_t)K2 * BJB + (col - BJB);
+      int64_t r = rb + row;
+      Ps[row][col] = (v < J.k && r < r1) ? base[v * J.L + r] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k0 = 0; k0 < 64; k0 += 4) {
+      double a = Ps[k0 + (lane >> 4)][16 * ti + (lane & 15)];
+      double b = Ps[k0 + (lane >> 4)][16 * tj + (lane & 15)];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double* out = Gpart + (int64_t)item * (BJ2 * BJ2);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    int r = 16 * ti + (lane >> 4) + 4 * e, c = 16 * tj + (lane & 15);
+    out[r * BJ2 + c] = acc[e];
+  }
+}
+
+__device__ __forceinline__ void brot(const double (*G)[BJ2 + 1], int x, int px, double tol,
+                                     double& a, double& b, bool& rot) {
+  int l = min(x, px), h = max(x, px);
+  double gpq = G[l][h], gpp = G[l][l], gqq = G[h][h];
+  double c = 1.0, sn = 0.0;
+  rot = false;
+  if (gpq != 0.0 && fabs(gpq) > tol * sqrt(fmax(gpp * gqq, 0.0))) {
+    double th = (gqq - gpp) / (2.0 * gpq);
+    double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
+    c = 1.0 / sqrt(1.0 + t * t);
+    sn = t * c;
+    rot = true;
+  }
+  a = c;
+  b = x == l ? -sn : sn;
+}
+
+// eig items are (job, pair) in job-major order; Gpart items for (job, pair,
+// chunk) start at gram_base + pair * nch.
+__global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart,
+                                                const BJob* __restrict__ jobs,
+                                                const int* __restrict__ e_job,
+                                                const int* __restrict__ e_pair,
+                                                const int* __restrict__ done,
+                                                double* __restrict__ Qall, int* __restrict__ skip,
+                                                unsigned long long* __restrict__ offmax) {
+  __shared__ double G[2][BJ2][BJ2 + 1];
+  __shared__ double Q[2][BJ2][BJ2 + 1];
+  __shared__ double red[4];
+  __shared__ int rotated[3];
+  const int item = blockIdx.x, tid = threadIdx.x;
+  const int j = e_job[item];
+  if (done[j]) return;
+  const BJob J = jobs[j];
+  const int pi = e_pair[item];
+  for (int e = tid; e < BJ2 * BJ2; e += 256) {
+    double acc = 0.0;
+    for (int c = 0; c < J.nch; ++c)
+      acc += Gpart[((int64_t)J.gram_base + (int64_t)pi * J.nch + c) * (BJ2 * BJ2) + e];
+    G[0][e / BJ2][e % BJ2] = acc;
+    Q[0][e / BJ2][e % BJ2] = (e / BJ2 == e % BJ2) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  double off = 0.0;
+  for (int e = tid; e < BJ2 * BJ2; e += 256) {
+    int p = e / BJ2, q = e % BJ2;
+    if (p < q) {
+      double d = G[0][p][p] * G[0][q][q];
+      if (d > 0.0) off = fmax(off, fabs(G[0][p][q]) / sqrt(d));
+    }
+  }
+  off = wave_max(off);
+  if ((tid & 63) == 0) red[tid >> 6] = off;
+  __syncthreads();
+  off = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  if (tid == 0) atomicMax(offmax + j, (unsigned long long)__double_as_longlong(off));
+  if (off <= J.tol) {
+    if (tid == 0) skip[item] = 1;
+    return;
+  }
+  if (tid == 0) skip[item] = 0;
+  int cur = 0;
+  if (tid == 0) rotated[0] = rotated[1] = rotated[2] = 0;
+  __syncthreads();
+  const int rp = tid >> 4, cp = tid & 15;
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    const int slot = sweep % 3;
+    if (tid == 0) rotated[(sweep + 1) % 3] = 0;
+    for (int r = 0; r < BJ2 - 1; ++r) {
+      const int nxt = cur ^ 1;
+      int x0 = rp == 0 ? 0 : 1 + (rp - 1 + r) % (BJ2 - 1);
+      int x1 = 1 + (BJ2 - 2 - rp + r) % (BJ2 - 1);
+      int y0 = cp == 0 ? 0 : 1 + (cp - 1 + r) % (BJ2 - 1);
+      int y1 = 1 + (BJ2 - 2 - cp + r) % (BJ2 - 1);
+      double ax0, bx0, ax1, bx1, ay0, by0, ay1, by1;
+      bool rx, ry;
+      brot(G[cur], x0, x1, J.tol, ax0, bx0, rx);
+      brot(G[cur], x1, x0, J.tol, ax1, bx1, rx);
+      brot(G[cur], y0, y1, J.tol, ay0, by0, ry);
+      brot(G[cur], y1, y0, J.tol, ay1, by1, ry);
+      const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
+      const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
+      const double r00 = ax0 * g00 + bx0 * g10, r01 = ax0 * g01 + bx0 * g11;
+      const double r10 = ax1 * g10 + bx1 * g00, r11 = ax1 * g11 + bx1 * g01;
+      G[nxt][x0][y0] = ay0 * r00 + by0 * r01;
+      G[nxt][x0][y1] = ay1 * r01 + by1 * r00;
+      G[nxt][x1][y0] = ay0 * r10 + by0 * r11;
+      G[nxt][x1][y1] = ay1 * r11 + by1 * r10;
+      const double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
+      const double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
+      Q[nxt][x0][y0] = ay0 * q00 + by0 * q01;
+      Q[nxt][x0][y1] = ay1 * q01 + by1 * q00;
+      Q[nxt][x1][y0] = ay0 * q10 + by0 * q11;
+      Q[nxt][x1][y1] = ay1 * q11 + by1 * q10;
+      if (rp == cp && rx) rotated[slot] = 1;
+      cur = nxt;
+      __syncthreads();
+    }
+    const int again = rotated[slot];
+    if (!again) break;
+  }
+  double* out = Qall + (int64_t)item * (BJ2 * BJ2);
+  for (int e = tid; e < BJ2 * BJ2; e += 256) out[e] = Q[cur][e / BJ2][e % BJ2];
+}
+
+// apply items are (job, pair, chunk) over the L + k rows of [W; Z]
+__global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, double* __restrict__ Zb,
+                                                  const BJob* __restrict__ jobs,
+                                                  const int* __restrict__ a_job,
+                                                  const int* __restrict__ a_pair,
+                                                  const int* __restrict__ a_chunk,
+                                                  const int* __restrict__ done, int g,
+                                                  const double* __restrict__ Qall,
+                                                  const int* __restrict__ skip) {
+  __shared__ double Q[BJ2][BJ2];
+  const int item = blockIdx.x;
+  const int j = a_job[item];
+  if (done[j]) return;
+  const BJob J = jobs[j];
+  const int pi = a_pair[item], ch = a_chunk[item];
+  const int eidx = J.eig_base + pi;
+  if (skip[eidx]) return;
+  const int s = g % (J.nblk - 1);
+  const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
+  for (int e = threadIdx.x; e < BJ2 * BJ2; e += 256) Q[e / BJ2][e % BJ2] = Qall[(int64_t)eidx * BJ2 * BJ2 + e];
+  __syncthreads();
+  double* base;
+  int64_t ld, r;
+  if (ch < J.nch) {
+    base = Wv + J.wv; ld = J.L; r = (int64_t)ch * BCH + threadIdx.x;
+    if (r >= J.L) return;
+  } else {
+    base = Zb + J.z; ld = J.k; r = (int64_t)(ch - J.nch) * BCH + threadIdx.x;
+    if (r >= J.k) return;
+  }
+  double x[BJ2];
+#pragma unroll
+  for (int c = 0; c < BJ2; ++c) {
+    int64_t v = c < BJB ? (int64_t)I * BJB + c : (int64_t)K2 * BJB + (c - BJB);
+    x[c] = v < J.k ? base[v * ld + r] : 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < BJ2; ++c) {
+    int64_t v = c < BJB ? (int64_t)I * BJB + c : (int64_t)K2 * BJB + (c - BJB);
+    if (v >= J.k) continue;
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < BJ2; ++i) acc = fma(x[i], Q[i][c], acc);
+    base[v * ld + r] = acc;
+  }
+}
+
+// After global step g: retire jobs whose sweep just ended with off <= tol.
+__global__ void bjac_sweep_end(const BJob* __restrict__ jobs, int njobs, int g, int max_sweeps,
+                               int* __restrict__ done, int* __restrict__ sweeps,
+                               unsigned long long* __restrict__ offmax, int* __restrict__ active) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= njobs || done[j]) return;
+  const BJob J = jobs[j];
+  if ((g + 1) % (J.nblk - 1) != 0) return;
+  double off = __longlong_as_double((long long)offmax[j]);
+  offmax[j] = 0ull;
+  int sw = ++sweeps[j];
+  if (off <= J.tol) {
+    done[j] = 1;
+    atomicSub(active, 1);
+  } else if (sw >= max_sweeps) {
+    done[j] = 2;
+    atomicSub(active, 1);
+  }
+}
+
+__global__ void binit_identity(double* __restrict__ Zb, const BJob* __restrict__ jobs) {
+  const BJob J = jobs[blockIdx.y];
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)J.k * J.k) return;
+  Zb[J.z + idx] = (idx / J.k == idx % J.k) ? 1.0 : 0.0;
+}
+
+__global__ void __launch_bounds__(256) bvec_norms(const double* __restrict__ Wv,
+                                                  const BJob* __restrict__ jobs,
+                                                  double* __restrict__ sig, const int64_t* sbase) {
+  __shared__ double red[4];
+  const BJob J = jobs[blockIdx.y];
+  const int v = blockIdx.x;
+  if (v >= J.k) return;
+  double acc = 0.0;
+  for (int64_t r = threadIdx.x; r < J.L; r += 256) {
+    double x = Wv[J.wv + (int64_t)v * J.L + r];
+    acc = fma(x, x, acc);
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) sig[sbase[blockIdx.y] + v] = sqrt(red[0] + red[1] + red[2] + red[3]);
+}
+
+// per job: sort descending, count kept = #{sigma/sigma_0 > thr}, <= chi
+__global__ void __launch_bounds__(1024) bsort_rank(const double* __restrict__ sig,
+                                                   const BJob* __restrict__ jobs,
+                                                   const int64_t* __restrict__ sbase,
+                                                   int* __restrict__ perm,
+                                                   double* __restrict__ Ssorted, double thr,
+                                                   int chi, int* __restrict__ kept) {
+  __shared__ int cnt;
+  const int j = blockIdx.x;
+  const BJob J = jobs[j];
+  const double* s = sig + sbase[j];
+  double smax = 0.0;
+  for (int i = 0; i < J.k; ++i) smax = fmax(smax, s[i]);
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int local = 0;
+  for (int v = threadIdx.x; v < J.k; v += 1024) {
+    double sv = s[v];
+    int rank = 0;
+    for (int i = 0; i < J.k; ++i) rank += (s[i] > sv || (s[i] == sv && i < v)) ? 1 : 0;
+    perm[sbase[j] + rank] = v;
+    Ssorted[sbase[j] + rank] = sv;
+    if (smax > 0.0 && sv / smax > thr) ++local;
+  }
+  atomicAdd(&cnt, local);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int k = cnt;
+    if (chi > 0 && k > chi) k = chi;
+    kept[j] = k;
+  }
+}
+
+struct OJob {
+  float* U;    // m x k row-major (FP32 output)
+  float* Vt;   // k x n row-major
+  double* S;   // k
+  int32_t m, n;
+  int32_t tall;
+};
+
+__global__ void bout(const double* __restrict__ Wv, const double* __restrict__ Zb,
+                     const BJob* __restrict__ jobs, const OJob* __restrict__ outs,
+                     const int64_t* __restrict__ sbase, const int* __restrict__ perm,
+                     const double* __restrict__ Ss) {
+  const int j = blockIdx.y;
+  const BJob J = jobs[j];
+  const OJob O = outs[j];
+  const int64_t k = J.k;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tot = (int64_t)O.m * k + k * O.n;
+  const int* pm = perm + sbase[j];
+  const double* S = Ss + sbase[j];
+  for (; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    if (idx < (int64_t)O.m * k) {
+      int64_t i = idx / k, c = idx - i * k;
+      int64_t src = pm[c];
+      double v = O.tall ? (S[c] > 0.0 ? Wv[J.wv + src * J.L + i] / S[c] : 0.0)
+                        : Zb[J.z + src * k + i];
+      O.U[idx] = (float)v;
+    } else {
+      int64_t e = idx - (int64_t)O.m * k;
+      int64_t r = e / O.n, c = e - r * O.n;
+      int64_t src = pm[r];
+      double v = O.tall ? Zb[J.z + src * k + c]
+                        : (S[r] > 0.0 ? Wv[J.wv + src * J.L + c] / S[r] : 0.0);
+      O.Vt[e] = (float)v;
+    }
+  }
+  if (blockIdx.x == 0)
+    for (int64_t c = threadIdx.x; c < k; c += blockDim.x) O.S[c] = S[c];
+}
+
+// ---------------------------------------------------------------- autotune
+std::mutex g_tune_mu;
+std::map<int, int> g_tune;  // size bucket -> FR (2: 64x64 tiles, 4: 128x128)
+
+static int bucket_of(int M, int N) {
+  int x = std::max(M, N);
+  return x <= 128 ? 0 : x <= 256 ? 1 : x <= 512 ? 2 : 3;
+}
+
+static int launch_grouped(int FR, const std::vector<int>& ids, const std::vector<GJob>& hjobs,
+                          GJob* djobs, int* dtile, int* dids, hipStream_t st) {
+  if (ids.empty()) return TN_OK;
+  const int BT = 32 * FR;
+  std::vector<int> base(ids.size());
+  int tot = 0;
+  for (size_t i = 0; i < ids.size(); ++i) {
+    base[i] = tot;
+    const GJob& g = hjobs[ids[i]];
+    tot += (int)(ceil_div(g.M, BT) * ceil_div(g.N, BT));
+  }
+  TN_HIP(hipMemcpyAsync(dtile, base.data(), sizeof(int) * base.size(), hipMemcpyHostToDevice, st));
+  TN_HIP(hipMemcpyAsync(dids, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, st));
+  TN_HIP(hipStreamSynchronize(st));  // host tables are locals
+  if (tot == 0) return TN_OK;
+  if (FR == 2)
+    hipLaunchKernelGGL(gemm_grouped_f32<2>, dim3(tot), dim3(256), 0, st, djobs, dtile, dids,
+                       (int)ids.size());
+  else
+    hipLaunchKernelGGL(gemm_grouped_f32<4>, dim3(tot), dim3(256), 0, st, djobs, dtile, dids,
+                       (int)ids.size());
+  TN_CHECK_LAUNCH();
+  return TN_OK;
+}
+
+static double time_grouped(int FR, const std::vector<int>& ids, const std::vector<GJob>& hjobs,
+                           GJob* djobs, int* dtile, int* dids, hipStream_t st) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 1e30;
+  (void)hipEventRecord(a, st);
+  int rc = launch_grouped(FR, ids, hjobs, djobs, dtile, dids, st);
+  (void)hipEventRecord(b, st);
+  (void)hipEventSynchronize(b);
+  float ms = 1e30f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc == TN_OK ? ms : 1e30;
+}
+
+}  // namespace tn
+
+using namespace tn;
+
+extern "C" {
+
+size_t tn_batched_truncate_workspace(const tn_trunc_job* jobs, int njobs) {
+  if (njobs <= 0) return 256;
+  size_t wv = 0, z = 0, gi = 0, ei = 0, ai = 0, sv = 0;
+  for (int j = 0; j < njobs; ++j) {
+    int64_t k = std::min(jobs[j].m, jobs[j].n), L = std::max(jobs[j].m, jobs[j].n);
+    int nblk = (int)ceil_div(std::max<int64_t>(k, 1), BJB);
+    nblk += nblk % 2;
+    int np = nblk / 2;
+    wv += (size_t)(L * k);
+    z += (size_t)(k * k);
+    gi += (size_t)np * ceil_div(L, BCH);
+    ei += (size_t)np;
+    ai += (size_t)np * (ceil_div(L, BCH) + ceil_div(k, BCH));
+    sv += (size_t)k;
+  }
+  Arena a{nullptr, 0, 0};
+  a.take<double>(wv);
+  a.take<double>(z);
+  a.take<double>(gi * BJ2 * BJ2);
+  a.take<double>(ei * BJ2 * BJ2);
+  a.take<int>(ei);
+  a.take<int>(gi * 3);
+  a.take<int>(ei * 2);
+  a.take<int>(ai * 3);
+  a.take<BJob>((size_t)njobs);
+  a.take<OJob>((size_t)njobs);
+  a.take<GJob>((size_t)njobs);
+  a.take<int>((size_t)njobs * 2);  // tile base + ids
+  a.take<int>((size_t)njobs * 4);  // done, sweeps, kept, active
+  a.take<unsigned long long>((size_t)njobs);
+  a.take<double>(sv * 2);
+  a.take<int>(sv);
+  a.take<int64_t>((size_t)njobs);
+  return a.used + 4096;
+}
+
+int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshold, int chi_max,
+                        int max_sweeps, int* kept_out, int* sweeps_out, void* ws, size_t ws_bytes,
+                        void* stream) {
+  TN_REQUIRE(njobs >= 0 && jobs != nullptr, "tn_batched_truncate: bad job list");
+  if (njobs == 0) return TN_OK;
+  TN_REQUIRE(ws && ws_bytes >= tn_batched_truncate_workspace(jobs, njobs),
+             "tn_batched_truncate: workspace too small");
+  if (max_sweeps <= 0) max_sweeps = 40;
+  hipStream_t st = as_stream(stream);
+  // ---- host tables
+  std::vector<BJob> bj(njobs);
+  std::vector<OJob> oj(njobs);
+  std::vector<GJob> gj(njobs);
+  std::vector<int> g_job, g_pair, g_chunk, e_job, e_pair, a_job, a_pair, a_chunk;
+  std::vector<int64_t> sbase(njobs);
+  int64_t wv = 0, z = 0, sv = 0;
+  int maxsteps = 1, maxk = 1;
+  for (int j = 0; j < njobs; ++j) {
+    const tn_trunc_job& J = jobs[j];
+    TN_REQUIRE(J.m > 0 && J.n > 0 && J.k > 0 && J.L && J.R && J.U && J.S && J.Vt,
+               "tn_batched_truncate: job %d is invalid", j);
+    TN_REQUIRE(J.m <= (1 << 20) && J.n <= (1 << 20) && J.k <= (1 << 20),
+               "tn_batched_truncate: job %d too large", j);
+    const int64_t k = std::min(J.m, J.n), L = std::max(J.m, J.n);
+    BJob b{};
+    b.wv = wv;
+    b.z = z;
+    b.L = (int)L;
+    b.k = (int)k;
+    b.nblk = (int)ceil_div(k, BJB);
+    b.nblk += b.nblk % 2;
+    b.nch = (int)ceil_div(L, BCH);
+    b.nchz = (int)ceil_div(k, BCH);
+    b.gram_base = (int)g_job.size();
+    b.eig_base = (int)e_job.size();
+    b.tol = std::max(1e-15, (double)L * 2.220446049250313e-16);
+    for (int p = 0; p < b.nblk / 2; ++p) {
+      for (int c = 0; c < b.nch; ++c) { g_job.push_back(j); g_pair.push_back(p); g_chunk.push_back(c); }
+      e_job.push_back(j);
+      e_pair.push_back(p);
+      for (int c = 0; c < b.nch + b.nchz; ++c) { a_job.push_back(j); a_pair.push_back(p); a_chunk.push_back(c); }
+    }
+    bj[j] = b;
+    sbase[j] = sv;
+    const bool tall = J.m >= J.n;
+    gj[j] = GJob{(const float*)J.L, (const float*)J.R, nullptr, (int)J.m, (int)J.n, (int)J.k,
+                 tall ? 1 : (int64_t)J.n, tall ? (int64_t)J.m : 1};
+    oj[j] = OJob{(float*)J.U, (float*)J.Vt, J.S, (int)J.m, (int)J.n, tall ? 1 : 0};
+    wv += L * k;
+    z += k * k;
+    sv += k;
+    maxsteps = std::max(maxsteps, b.nblk - 1);
+    maxk = std::max<int>(maxk, (int)k);
+  }
+  const int ngi = (int)g_job.size(), nei = (int)e_job.size(), nai = (int)a_job.size();
+  // ---- device workspace
+  Arena a{(char*)ws, ws_bytes, 0};
+  double* dWv = a.take<double>((size_t)wv);
+  double* dZ = a.take<double>((size_t)z);
+  double* dG = a.take<double>((size_t)ngi * BJ2 * BJ2);
+  double* dQ = a.take<double>((size_t)nei * BJ2 * BJ2);
+  int* dskip = a.take<int>((size_t)nei);
+  int* dgt = a.take<int>((size_t)ngi * 3);
+  int* det = a.take<int>((size_t)nei * 2);
+  int* dat = a.take<int>((size_t)nai * 3);
+  BJob* dbj = a.take<BJob>((size_t)njobs);
+  OJob* doj = a.take<OJob>((size_t)njobs);
+  GJob* dgj = a.take<GJob>((size_t)njobs);
+  int* dtile = a.take<int>((size_t)njobs);
+  int* dids = a.take<int>((size_t)njobs);
+  int* ddone = a.take<int>((size_t)njobs);
+  int* dsweeps = a.take<int>((size_t)njobs);
+  int* dkept = a.take<int>((size_t)njobs);
+  int* dactive = a.take<int>((size_t)njobs);
+  unsigned long long* doff = a.take<unsigned long long>((size_t)njobs);
+  double* dsig = a.take<double>((size_t)sv);
+  double* dS = a.take<double>((size_t)sv);
+  int* dperm = a.take<int>((size_t)sv);
+  int64_t* dsbase = a.take<int64_t>((size_t)njobs);
+  TN_REQUIRE(dsbase != nullptr, "tn_batched_truncate: workspace too small");
+  for (int j = 0; j < njobs; ++j) gj[j].C = dWv + bj[j].wv;
+  std::vector<int> gt(ngi * 3), et(nei * 2), at(nai * 3);
+  for (int i = 0; i < ngi; ++i) { gt[i] = g_job[i]; gt[ngi + i] = g_pair[i]; gt[2 * ngi + i] = g_chunk[i]; }
+  for (int i = 0; i < nei; ++i) { et[i] = e_job[i]; et[nei + i] = e_pair[i]; }
+  for (int i = 0; i < nai; ++i) { at[i] = a_job[i]; at[nai + i] = a_pair[i]; at[2 * nai + i] = a_chunk[i]; }
+  TN_HIP(hipMemcpyAsync(dgt, gt.data(), sizeof(int) * gt.size(), hipMemcpyHostToDevice, st));
+  TN_HIP(hipMemcpyAsync(det, et.data(), sizeof(int) * et.size(), hipMemcpyHostToDevice, st));
+  TN_HIP(hipMemcpyAsync(dat, at.data(), sizeof(int) * at.size(), hipMemcpyHostToDevice, st));
+  TN_HIP(hipMemcpyAsync(dbj, bj.data(), sizeof(BJob) * njobs, hipMemcpyHostToDevice, st));
+  TN_HIP(hipMemcpyAsync(doj, oj.data(), sizeof(OJob) * njobs, hipMemcpyHostToDevice, st));
+  TN_HIP(hipMemcpyAsync(dgj, gj.data(), sizeof(GJob) * njobs, hipMemcpyHostToDevice, st));
+  TN_HIP(hipMemcpyAsync(dsbase, sbase.data(), sizeof(int64_t) * njobs, hipMemcpyHostToDevice, st));
+  TN_HIP(hipMemsetAsync(ddone, 0, sizeof(int) * njobs, st));
+  TN_HIP(hipMemsetAsync(dsweeps, 0, sizeof(int) * njobs, st));
+  TN_HIP(hipMemsetAsync(doff, 0, sizeof(unsigned long long) * njobs, st));
+  int h_active = njobs;
+  TN_HIP(hipMemcpyAsync(dactive, &h_active, sizeof(int), hipMemcpyHostToDevice, st));
+
+  // ---- 1. grouped contraction, tile shape per size bucket (autotuned once)
+  {
+    std::vector<int> byb[4];
+    for (int j = 0; j < njobs; ++j) byb[bucket_of(gj[j].M, gj[j].N)].push_back(j);
+    std::vector<int> use2, use4;
+    for (int bkt = 0; bkt < 4; ++bkt) {
+      if (byb[bkt].empty()) continue;
+      int fr;
+      {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(bkt);
+        fr = it == g_tune.end() ? 0 : it->second;
+      }
+      if (fr == 0) {
+        // time both shapes on this bucket (results are overwritten by the real launch)
+        double t2 = time_grouped(2, byb[bkt], gj, dgj, dtile, dids, st);
+        double t4 = time_grouped(4, byb[bkt], gj, dgj, dtile, dids, st);
+        fr = t2 <= t4 ? 2 : 4;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[bkt] = fr;
+      }
+      (fr == 2 ? use2 : use4).insert((fr == 2 ? use2 : use4).end(), byb[bkt].begin(), byb[bkt].end());
+    }
+    {
+      ProfScope p(PROF_GEMM_F32, st, 0.0, 0.0);
+      TN_TRY(launch_grouped(2, use2, gj, dgj, dtile, dids, st));
+    }
+    if (!use4.empty()) {
+      // second grouped launch needs its own tables: the first launch's are in use
+      TN_HIP(hipStreamSynchronize(st));
+      ProfScope p(PROF_GEMM_F32, st, 0.0, 0.0);
+      TN_TRY(launch_grouped(4, use4, gj, dgj, dtile, dids, st));
+    }
+  }
+  // ---- 2. batched Jacobi
+  hipLaunchKernelGGL(binit_identity, dim3((unsigned)ceil_div((int64_t)maxk * maxk, 256), njobs),
+                     dim3(256), 0, st, dZ, dbj);
+  TN_CHECK_LAUNCH();
+  const int poll = std::max(8, maxsteps);
+  int g = 0;
+  for (; g < maxsteps * max_sweeps; ++g) {
+    {
+      ProfScope p(PROF_JAC_GRAM, st, 0.0, 0.0);
+      hipLaunchKernelGGL(bjac_gram, dim3(ngi), dim3(256), 0, st, dWv, dbj, dgt, dgt + ngi,
+                         dgt + 2 * ngi, ddone, g, dG);
+    }
+    {
+      ProfScope p(PROF_JAC_EIG, st, 0.0, 0.0);
+      hipLaunchKernelGGL(bjac_eig, dim3(nei), dim3(256), 0, st, dG, dbj, det, det + nei, ddone, dQ,
+                         dskip, doff);
+    }
+    {
+      ProfScope p(PROF_JAC_APPLY, st, 0.0, 0.0);
+      hipLaunchKernelGGL(bjac_apply, dim3(nai), dim3(256), 0, st, dWv, dZ, dbj, dat, dat + nai,
+                         dat + 2 * nai, ddone, g, dQ, dskip);
+    }
+    hipLaunchKernelGGL(bjac_sweep_end, dim3((unsigned)ceil_div(njobs, 256)), dim3(256), 0, st, dbj,
+                       njobs, g, max_sweeps, ddone, dsweeps, doff, dactive);
+    TN_CHECK_LAUNCH();
+    if ((g + 1) % poll == 0) {
+      TN_HIP(hipMemcpyAsync(&h_active, dactive, sizeof(int), hipMemcpyDeviceToHost, st));
+      TN_HIP(hipStreamSynchronize(st));
+      if (h_active <= 0) break;
+    }
+  }
+  // ---- 3. sort, rank, outputs
+  hipLaunchKernelGGL(bvec_norms, dim3((unsigned)maxk, njobs), dim3(256), 0, st, dWv, dbj, dsig,
+                     dsbase);
+  hipLaunchKernelGGL(bsort_rank, dim3(njobs), dim3(1024), 0, st, dsig, dbj, dsbase, dperm, dS,
+                     rel_threshold, chi_max, dkept);
+  hipLaunchKernelGGL(bout, dim3(256, njobs), dim3(256), 0, st, dWv, dZ, dbj, doj, dsbase, dperm, dS);
+  TN_CHECK_LAUNCH();
+  std::vector<int> done(njobs);
+  if (kept_out)
+    TN_HIP(hipMemcpyAsync(kept_out, dkept, sizeof(int) * njobs, hipMemcpyDeviceToHost, st));
+  TN_HIP(hipMemcpyAsync(done.data(), ddone, sizeof(int) * njobs, hipMemcpyDeviceToHost, st));
+  if (sweeps_out)
+    TN_HIP(hipMemcpyAsync(sweeps_out, dsweeps, sizeof(int) * njobs, hipMemcpyDeviceToHost, st));
+  TN_HIP(hipStreamSynchronize(st));
+  for (int j = 0; j < njobs; ++j)
+    if (done[j] != 1) {
+      set_error(TN_ERR_NOCONV, "tn_batched_truncate: job %d did not converge", j);
+      return TN_ERR_NOCONV;
+    }
+  return TN_OK;
+}
+
+}  // extern "C"

@@ -181,7 +181,9 @@ __global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart,
   if (tid == 0) rotated[0] = rotated[1] = rotated[2] = 0;
   __syncthreads();
   const int rp = tid >> 4, cp = tid & 15;
-  for (int sweep = 0; sweep < 8; ++sweep) {
+  // At most two inner sweeps: the outer block iteration does not need each
+  // pair fully diagonalised, and late pairs are nearly diagonal already.
+  for (int sweep = 0; sweep < 2; ++sweep) {
     const int slot = sweep % 3;
     if (tid == 0) rotated[(sweep + 1) % 3] = 0;
     for (int r = 0; r < J2 - 1; ++r) {
