@@ -121,6 +121,25 @@ int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_
            void* U, double* S, void* Vt, int dtype, double tol, int max_sweeps, int* sweeps_out,
            void* ws, size_t ws_bytes, void* stream);
 
+/* ---- PSD eigendecomposition (truncation from a bond Gram matrix) ---------
+ * K = U diag(lam) U^T for a symmetric positive semi-definite n x n matrix K
+ * (decoded as in tn_qr; FP64 or FP32 input).  U: n x n row-major FP64
+ * (column j = eigenvector j), lam: n FP64, descending.  Used where only the
+ * Gram matrix K = C^T E C of a bond is formed: its eigenvectors are the left
+ * singular vectors the reference's truncating SVD computes
+ * (onedim_core.py:317 -> tensor.py:915) and lam = sigma^2.
+ * Method: static diagonal pivot, shift delta = 8 sqrt(n) eps_in tr(K) with
+ * eps_in the unit roundoff of dtype (eigenvectors unchanged; pass a K formed
+ * in FP32 as TN_F32 so the shift covers its rounding), blocked Cholesky
+ * (a pivot below delta/2 drops its column), one-sided block Jacobi on the
+ * Cholesky factor's columns (FP64 rotations); lam = sigma^2 - delta.  tol <= 0 ->
+ * n * eps; max_sweeps <= 0 -> 40; *sweeps_out (host, may be NULL).
+ * Returns TN_ERR_NOCONV if not converged. */
+size_t tn_eigh_psd_workspace(int64_t n);
+int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes* k_cols, int64_t n,
+                double* U, double* lam, int dtype, double tol, int max_sweeps, int* sweeps_out,
+                void* ws, size_t ws_bytes, void* stream);
+
 /* ---- batched uneven contraction + truncation --------------------------
  * No single reference call: each job is what the reference does with one
  * tensordot (tensor.py:735) followed by truncated_svd (tensor.py:1112) with a

@@ -193,7 +193,8 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
   __shared__ double G[2][BJ2][BJ2 + 1];
   __shared__ double Q[2][BJ2][BJ2 + 1];
   __shared__ double red[4];
-  __shared__ int rotated[3];
+  __shared__ int rotated[1];
+  __shared__ double cs[BJ2 / 2][2];
   const int item = blockIdx.x, tid = threadIdx.x;
   const int j = e_job[item];
   if (done[j]) return;
@@ -226,44 +227,45 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
   }
   if (tid == 0) skip[item] = 0;
   int cur = 0;
-  if (tid == 0) rotated[0] = rotated[1] = rotated[2] = 0;
+  if (tid == 0) rotated[0] = 0;
   __syncthreads();
   const int rp = tid >> 4, cp = tid & 15;
-  for (int sweep = 0; sweep < 2; ++sweep) {
-    const int slot = sweep % 3;
-    if (tid == 0) rotated[(sweep + 1) % 3] = 0;
-    for (int r = 0; r < BJ2 - 1; ++r) {
-      const int nxt = cur ^ 1;
-      int x0 = rp == 0 ? 0 : 1 + (rp - 1 + r) % (BJ2 - 1);
-      int x1 = 1 + (BJ2 - 2 - rp + r) % (BJ2 - 1);
-      int y0 = cp == 0 ? 0 : 1 + (cp - 1 + r) % (BJ2 - 1);
-      int y1 = 1 + (BJ2 - 2 - cp + r) % (BJ2 - 1);
-      double ax0, bx0, ax1, bx1, ay0, by0, ay1, by1;
-      bool rx, ry;
-      brot(G[cur], x0, x1, J.tol, ax0, bx0, rx);
-      brot(G[cur], x1, x0, J.tol, ax1, bx1, rx);
-      brot(G[cur], y0, y1, J.tol, ay0, by0, ry);
-      brot(G[cur], y1, y0, J.tol, ay1, by1, ry);
-      const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
-      const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
-      const double r00 = ax0 * g00 + bx0 * g10, r01 = ax0 * g01 + bx0 * g11;
-      const double r10 = ax1 * g10 + bx1 * g00, r11 = ax1 * g11 + bx1 * g01;
-      G[nxt][x0][y0] = ay0 * r00 + by0 * r01;
-      G[nxt][x0][y1] = ay1 * r01 + by1 * r00;
-      G[nxt][x1][y0] = ay0 * r10 + by0 * r11;
-      G[nxt][x1][y1] = ay1 * r11 + by1 * r10;
-      const double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
-      const double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
-      Q[nxt][x0][y0] = ay0 * q00 + by0 * q01;
-      Q[nxt][x0][y1] = ay1 * q01 + by1 * q00;
-      Q[nxt][x1][y0] = ay0 * q10 + by0 * q11;
-      Q[nxt][x1][y1] = ay1 * q11 + by1 * q10;
-      if (rp == cp && rx) rotated[slot] = 1;
-      cur = nxt;
-      __syncthreads();
+  // one inner sweep, two barriers per round: 16 threads compute the round's
+  // rotations, then every thread applies its 2x2 block of G' and Q'
+  for (int r = 0; r < BJ2 - 1; ++r) {
+    const int nxt = cur ^ 1;
+    if (tid < BJ2 / 2) {
+      int x0 = tid == 0 ? 0 : 1 + (tid - 1 + r) % (BJ2 - 1);
+      int x1 = 1 + (BJ2 - 2 - tid + r) % (BJ2 - 1);
+      double a0, b0;
+      bool rx;
+      brot(G[cur], x0, x1, J.tol, a0, b0, rx);
+      cs[tid][0] = a0;
+      cs[tid][1] = b0;
     }
-    const int again = rotated[slot];
-    if (!again) break;
+    __syncthreads();
+    int x0 = rp == 0 ? 0 : 1 + (rp - 1 + r) % (BJ2 - 1);
+    int x1 = 1 + (BJ2 - 2 - rp + r) % (BJ2 - 1);
+    int y0 = cp == 0 ? 0 : 1 + (cp - 1 + r) % (BJ2 - 1);
+    int y1 = 1 + (BJ2 - 2 - cp + r) % (BJ2 - 1);
+    const double cx = cs[rp][0], bX0 = cs[rp][1], bX1 = -cs[rp][1];
+    const double cy = cs[cp][0], bY0 = cs[cp][1], bY1 = -cs[cp][1];
+    const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
+    const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
+    const double r00 = cx * g00 + bX0 * g10, r01 = cx * g01 + bX0 * g11;
+    const double r10 = cx * g10 + bX1 * g00, r11 = cx * g11 + bX1 * g01;
+    G[nxt][x0][y0] = cy * r00 + bY0 * r01;
+    G[nxt][x0][y1] = cy * r01 + bY1 * r00;
+    G[nxt][x1][y0] = cy * r10 + bY0 * r11;
+    G[nxt][x1][y1] = cy * r11 + bY1 * r10;
+    const double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
+    const double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
+    Q[nxt][x0][y0] = cy * q00 + bY0 * q01;
+    Q[nxt][x0][y1] = cy * q01 + bY1 * q00;
+    Q[nxt][x1][y0] = cy * q10 + bY0 * q11;
+    Q[nxt][x1][y1] = cy * q11 + bY1 * q10;
+    cur = nxt;
+    __syncthreads();
   }
   double* out = Qall + (int64_t)item * (BJ2 * BJ2);
   for (int e = tid; e < BJ2 * BJ2; e += 256) out[e] = Q[cur][e / BJ2][e % BJ2];
@@ -377,10 +379,15 @@ __global__ void __launch_bounds__(1024) bsort_rank(const double* __restrict__ si
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   int local = 0;
+  // NaN ranks as -inf so that the ranks stay a permutation of 0..k-1
   for (int v = threadIdx.x; v < J.k; v += 1024) {
-    double sv = s[v];
+    const double sv = s[v];
+    const double kv = sv == sv ? sv : -INFINITY;
     int rank = 0;
-    for (int i = 0; i < J.k; ++i) rank += (s[i] > sv || (s[i] == sv && i < v)) ? 1 : 0;
+    for (int i = 0; i < J.k; ++i) {
+      const double ki = s[i] == s[i] ? s[i] : -INFINITY;
+      rank += (ki > kv || (ki == kv && i < v)) ? 1 : 0;
+    }
     perm[sbase[j] + rank] = v;
     Ssorted[sbase[j] + rank] = sv;
     if (smax > 0.0 && sv / smax > thr) ++local;

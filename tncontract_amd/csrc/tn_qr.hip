@@ -19,7 +19,7 @@
 namespace tn {
 
 constexpr int QR_NB = 64;
-constexpr int PT = 1024;  // panel threads
+constexpr int PT = 512;  // panel threads (2 waves per SIMD: 256 VGPRs per thread)
 constexpr int NWAVE = PT / 64;
 
 // Block-wide sum of nv (<= NV) values; every thread receives the totals.
@@ -48,17 +48,22 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], int nv, double* lds, 
   __syncthreads();
 }
 
-// Factor columns [c0, c0+w) (w <= W) over rows [c0, m) of W (column-major).
+// Factor columns [c0, c0+w) (w <= W) over rows [c0, m) of A (column-major,
+// ld), one workgroup, the panel in registers: thread t owns rows
+// c0 + t + PT*i.  Per column: one reduction for ||x_j|| (which also carries
+// the Gram entries v_a . v_{j-1}, a < j-1, of the compact-WY factor) and one
+// for the update dot products.
 template <int R, int W>
 __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, int64_t m,
                                                       int64_t ld, int64_t c0, int w,
                                                       double* __restrict__ tau,
                                                       double* __restrict__ Tp) {
-  constexpr int NP = W * (W - 1) / 2 > W ? W * (W - 1) / 2 : W;
-  __shared__ double lds[NWAVE * NP];
-  __shared__ double out[NP];
-  __shared__ double piv[W];  // pivot value of column j (one slot per column: no reuse race)
+  __shared__ double lds[NWAVE * W];
+  __shared__ double out[W];
+  __shared__ double piv[W];       // pivot value of each column (one slot each)
+  __shared__ double gsh[W][W];    // v_a . v_b, a < b
   __shared__ double taus[W];
+  __shared__ double Tsh[W][W];
   const int t = threadIdx.x;
   double x[R][W];
 #pragma unroll
@@ -67,118 +72,126 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
 #pragma unroll
     for (int j = 0; j < W; ++j) x[i][j] = (r < m && j < w) ? A[r + (c0 + j) * ld] : 0.0;
   }
-  // fully unrolled over the panel columns so x[][] stays in registers
+  // every index below is a compile-time constant after unrolling (no
+  // early exit), so x[][] stays in registers
 #pragma unroll
   for (int j = 0; j < W; ++j) {
-    if (j >= w) break;
-    const int64_t p = c0 + j;  // pivot row, owned by thread j (i = 0)
-    double s[1] = {0.0};
+    if (j < w) {
+      // vals[0] = ||x_j below the pivot||^2; vals[1 + a] = v_a . v_{j-1}, a < j-1
+      double vals[W];
 #pragma unroll
-    for (int i = 0; i < R; ++i) {
-      int64_t r = c0 + t + (int64_t)PT * i;
-      if (r > p && r < m) s[0] = fma(x[i][j], x[i][j], s[0]);
-    }
-    if (t == j) piv[j] = x[0][j];
-    block_sum<1>(s, 1, lds, out);
-    const double alpha = piv[j];
-    const double xnorm2 = s[0];
-    double tj = 0.0, scale = 0.0, beta = alpha;
-    if (xnorm2 > 0.0) {
-      double nrm = sqrt(alpha * alpha + xnorm2);
-      beta = alpha >= 0.0 ? -nrm : nrm;
-      tj = (beta - alpha) / beta;
-      scale = 1.0 / (alpha - beta);
-    }
-    // v below the pivot, beta on the diagonal
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-      int64_t r = c0 + t + (int64_t)PT * i;
-      if (r > p && r < m) x[i][j] *= scale;
-    }
-    if (t == j) x[0][j] = beta;
-    if (t == 0) taus[j] = tj;
-    // apply H_j = I - tau v v^T to the remaining panel columns
-    const int nrest = w - 1 - j;
-    if (nrest > 0 && tj != 0.0) {
-      double d[W];
-#pragma unroll
-      for (int q = 0; q < W; ++q) d[q] = 0.0;
+      for (int q = 0; q < W; ++q) vals[q] = 0.0;
+      const int64_t p = c0 + j;
 #pragma unroll
       for (int i = 0; i < R; ++i) {
         int64_t r = c0 + t + (int64_t)PT * i;
-        if (r > p && r < m) {
+        if (r < m) {
+          if (r > p) vals[0] = fma(x[i][j], x[i][j], vals[0]);
+          if (j >= 2 && r > c0 + j - 1) {
 #pragma unroll
-          for (int q = 0; q < W; ++q)
-            if (q > j) d[q] = fma(x[i][j], x[i][q], d[q]);
+            for (int a = 0; a < j - 1; ++a) vals[1 + a] = fma(x[i][a], x[i][j - 1], vals[1 + a]);
+          }
         }
       }
-      // pivot-row contribution (v_p = 1)
-      if (t == j) {
+      if (j >= 2 && t == j - 1) {
+        // row c0 + (j-1) belongs to thread j-1 (i = 0): v_{j-1} = 1 there
 #pragma unroll
-        for (int q = 0; q < W; ++q)
-          if (q > j) d[q] += x[0][q];
+        for (int a = 0; a < j - 1; ++a) vals[1 + a] += x[0][a];
       }
-      double dd[W];
+      if (t == j) piv[j] = x[0][j];
+      block_sum<W>(vals, j >= 2 ? j : 1, lds, out);
+      if (j >= 2 && t == 0) {
 #pragma unroll
-      for (int q = 0; q < W; ++q) dd[q] = d[q];
-      block_sum<W>(dd, w, lds, out);
+        for (int a = 0; a < j - 1; ++a) gsh[a][j - 1] = vals[1 + a];
+      }
+      const double alpha = piv[j];
+      const double xnorm2 = vals[0];
+      double tj = 0.0, scale = 0.0, beta = alpha;
+      if (xnorm2 > 0.0) {
+        double nrm = sqrt(alpha * alpha + xnorm2);
+        beta = alpha >= 0.0 ? -nrm : nrm;
+        tj = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+      }
 #pragma unroll
       for (int i = 0; i < R; ++i) {
         int64_t r = c0 + t + (int64_t)PT * i;
-        if (r >= p && r < m) {
-          double v = (r == p) ? 1.0 : x[i][j];
+        if (r > p && r < m) x[i][j] *= scale;
+      }
+      if (t == j) x[0][j] = beta;
+      if (t == 0) taus[j] = tj;
+      if (j + 1 < w && tj != 0.0) {
+        double d[W];
 #pragma unroll
-          for (int q = 0; q < W; ++q)
-            if (q > j && q < w) x[i][q] -= tj * dd[q] * v;
+        for (int q = 0; q < W; ++q) d[q] = 0.0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          int64_t r = c0 + t + (int64_t)PT * i;
+          if (r > p && r < m) {
+#pragma unroll
+            for (int q = j + 1; q < W; ++q) d[q] = fma(x[i][j], x[i][q], d[q]);
+          }
         }
+        if (t == j) {
+#pragma unroll
+          for (int q = j + 1; q < W; ++q) d[q] += x[0][q];
+        }
+        block_sum<W>(d, w, lds, out);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          int64_t r = c0 + t + (int64_t)PT * i;
+          if (r >= p && r < m) {
+            double v = (r == p) ? 1.0 : x[i][j];
+#pragma unroll
+            for (int q = j + 1; q < W; ++q)
+              if (q < w) x[i][q] -= tj * d[q] * v;
+          }
+        }
+      }
+    }
+  }
+  // Gram entries v_a . v_{w-1} of the last column (a < w-1)
+#pragma unroll
+  for (int b = 1; b < W; ++b) {
+    if (b == w - 1) {
+      double vals[W];
+#pragma unroll
+      for (int q = 0; q < W; ++q) vals[q] = 0.0;
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        int64_t r = c0 + t + (int64_t)PT * i;
+        if (r < m && r > c0 + b) {
+#pragma unroll
+          for (int a = 0; a < b; ++a) vals[a] = fma(x[i][a], x[i][b], vals[a]);
+        }
+      }
+      if (t == b) {
+#pragma unroll
+        for (int a = 0; a < b; ++a) vals[a] += x[0][a];
+      }
+      block_sum<W>(vals, b, lds, out);
+      if (t == 0) {
+#pragma unroll
+        for (int a = 0; a < b; ++a) gsh[a][b] = vals[a];
       }
     }
   }
   __syncthreads();
-  // compact-WY factor of the panel: G[a][b] = v_a . v_b (a < b)
-  double g[NP];
-#pragma unroll
-  for (int q = 0; q < NP; ++q) g[q] = 0.0;
-  {
-    int q = 0;
-#pragma unroll
-    for (int a = 0; a < W; ++a)
-#pragma unroll
-      for (int b = a + 1; b < W; ++b, ++q) {
-        if (b >= w) continue;
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-          int64_t r = c0 + t + (int64_t)PT * i;
-          if (r < m) {
-            double va = (r == c0 + a) ? 1.0 : (r > c0 + a ? x[i][a] : 0.0);
-            double vb = (r == c0 + b) ? 1.0 : (r > c0 + b ? x[i][b] : 0.0);
-            g[q] = fma(va, vb, g[q]);
-          }
-        }
-      }
-  }
-  block_sum<NP>(g, W * (W - 1) / 2, lds, out);
   if (t == 0) {
-    double T[W][W];
-#pragma unroll
+    // T (upper triangular, w x w): T[b][b] = tau_b, T[0:b, b] = -tau_b T[0:b,0:b] G[0:b, b]
     for (int a = 0; a < W; ++a)
-#pragma unroll
-      for (int b = 0; b < W; ++b) T[a][b] = 0.0;
-    int q0[W][W];
-    int q = 0;
-    for (int a = 0; a < W; ++a)
-      for (int b = a + 1; b < W; ++b) q0[a][b] = q++;
+      for (int b = 0; b < W; ++b) Tsh[a][b] = 0.0;
     for (int b = 0; b < w; ++b) {
-      T[b][b] = taus[b];
+      Tsh[b][b] = taus[b];
       for (int a = 0; a < b; ++a) {
         double acc = 0.0;
-        for (int c = a; c < b; ++c) acc += T[a][c] * g[q0[c][b]];
-        T[a][b] = -taus[b] * acc;
+        for (int c = a; c < b; ++c) acc += Tsh[a][c] * gsh[c][b];
+        Tsh[a][b] = -taus[b] * acc;
       }
     }
     for (int a = 0; a < w; ++a) {
       tau[c0 + a] = taus[a];
-      for (int b = 0; b < w; ++b) Tp[a * w + b] = T[a][b];
+      for (int b = 0; b < w; ++b) Tp[a * w + b] = Tsh[a][b];
     }
   }
 #pragma unroll
@@ -424,9 +437,9 @@ static int launch_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, dou
 // Pick the register-panel instantiation for m_rem rows; returns panel width.
 static int panel_width(int64_t mrem) {
   int64_t R = ceil_div(mrem, PT);
-  if (R <= 6) return 8;
-  if (R <= 12) return 4;
-  if (R <= 16) return 2;
+  if (R <= 10) return 8;
+  if (R <= 14) return 4;
+  if (R <= 24) return 2;
   return 1;
 }
 
@@ -435,18 +448,14 @@ static int run_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, double
   int64_t R = ceil_div(m - c0, PT);
   if (R <= 1) return launch_panel<1, 8>(A, m, ld, c0, w, tau, Tp, st);
   if (R <= 2) return launch_panel<2, 8>(A, m, ld, c0, w, tau, Tp, st);
-  if (R <= 3) return launch_panel<3, 8>(A, m, ld, c0, w, tau, Tp, st);
   if (R <= 4) return launch_panel<4, 8>(A, m, ld, c0, w, tau, Tp, st);
-  if (R <= 5) return launch_panel<5, 8>(A, m, ld, c0, w, tau, Tp, st);
   if (R <= 6) return launch_panel<6, 8>(A, m, ld, c0, w, tau, Tp, st);
-  if (R <= 8) return launch_panel<8, 4>(A, m, ld, c0, w, tau, Tp, st);
-  if (R <= 10) return launch_panel<10, 4>(A, m, ld, c0, w, tau, Tp, st);
-  if (R <= 12) return launch_panel<12, 4>(A, m, ld, c0, w, tau, Tp, st);
-  if (R <= 16) return launch_panel<16, 2>(A, m, ld, c0, w, tau, Tp, st);
-  if (R <= 24) return launch_panel<24, 1>(A, m, ld, c0, w, tau, Tp, st);
-  if (R <= 32) return launch_panel<32, 1>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 8) return launch_panel<8, 8>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 10) return launch_panel<10, 8>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 14) return launch_panel<14, 4>(A, m, ld, c0, w, tau, Tp, st);
+  if (R <= 24) return launch_panel<24, 2>(A, m, ld, c0, w, tau, Tp, st);
   if (R <= 48) return launch_panel<48, 1>(A, m, ld, c0, w, tau, Tp, st);
-  set_error(TN_ERR_UNSUPPORTED, "tn_qr: %lld rows exceed the register-panel limit (49152)",
+  set_error(TN_ERR_UNSUPPORTED, "tn_qr: %lld rows exceed the register-panel limit (24576)",
             (long long)(m - c0));
   return TN_ERR_UNSUPPORTED;
 }
@@ -535,12 +544,16 @@ int qr_form_q(const double* W, int64_t m, int64_t n, const QrWs& ws, double* Qw,
   for (int64_t b = nblk - 1; b >= 0; --b) {
     const int64_t j0 = b * QR_NB;
     const int nbo = (int)std::min<int64_t>(QR_NB, k - j0);
-    const int64_t mr = m - j0, nc = k - j0;
+    // with [I; 0] the columns left of j0 are still unit vectors that this
+    // block's reflectors (rows >= j0) leave alone; a general start matrix
+    // needs every column updated.
+    const int64_t c0 = init == nullptr ? j0 : 0;
+    const int64_t mr = m - j0, nc = k - c0;
     hipLaunchKernelGGL(qr_build_v_kernel, dim3((unsigned)ceil_div(mr * nbo, 256)), dim3(256), 0,
                        st, W, m, m, j0, nbo, ws.V);
     TN_CHECK_LAUNCH();
     const double* T = ws.Tall + b * QR_NB * QR_NB;
-    double* Qs = Qw + j0 + j0 * m;
+    double* Qs = Qw + j0 + c0 * m;
     TN_TRY(contract_launch(ws.V, mk({nbo}, {mr}), mk({mr}, {1}), Qs, mk({mr}, {1}),
                            mk({nc}, {m}), ws.Y, mk({nbo}, {nc}), mk({nc}, {1}), TN_F64,
                            TN_PREC_NATIVE, 1.0, 0.0, ws.part, ws.part_bytes, st));

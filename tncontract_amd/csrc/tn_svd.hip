@@ -47,9 +47,23 @@ __device__ __forceinline__ int64_t pair_vec(int c, int I, int J) {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+struct EigLds {
+  double G[2][J2][J2 + 1];
+  double Q[2][J2][J2 + 1];
+  double cs[J2 / 2][2];
+  double red[4];
+  int rotated[3];
+};
+__device__ void pair_eig(EigLds& sm, const double* __restrict__ Gpart, int nchunk, int pi,
+                         double tol, double* __restrict__ Qall, int* __restrict__ skip,
+                         unsigned long long* __restrict__ offmax, int nvI, int nvJ);
+
 __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, int64_t L,
                                                 int64_t nv, int s, int nblk, int nchunk,
-                                                double* __restrict__ Gpart) {
+                                                double* __restrict__ Gpart, int* counter,
+                                                double tol, double* __restrict__ Qall,
+                                                int* __restrict__ skip,
+                                                unsigned long long* __restrict__ offmax) {
   __shared__ double Ps[2][64][J2 + 1];
   const int pi = blockIdx.x, ch = blockIdx.y;
   int I, J;
@@ -101,6 +115,21 @@ __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, i
     int r = 16 * ti + (lane >> 4) + 4 * e, c = 16 * tj + (lane & 15);
     out[r * J2 + c] = acc[e];
   }
+  if (counter == nullptr) return;
+  // Last-arriving chunk block of the pair runs the pair eigensolve: release
+  // the partial device-wide, count arrivals, acquire before reading partials.
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) last = (atomicAdd(counter + pi, 1) == nchunk - 1) ? 1 : 0;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (tid == 0) counter[pi] = 0;  // ready for the next step
+  __shared__ EigLds sm;
+  pair_eig(sm, Gpart, nchunk, pi, tol, Qall, skip, offmax,
+           (int)max<int64_t>(0, min<int64_t>(JB, nv - (int64_t)I * JB)),
+           (int)max<int64_t>(0, min<int64_t>(JB, nv - (int64_t)J * JB)));
 }
 
 // Largest normalised off-diagonal over the pair, encoded for atomicMax.
@@ -136,15 +165,16 @@ __device__ __forceinline__ void rot_coef(const double (*G)[J2 + 1], int x, int p
   b = x == l ? -sn : sn;
 }
 
-__global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart, int nchunk,
-                                               int64_t nv, int s, int nblk, double tol,
-                                               double* __restrict__ Qall, int* __restrict__ skip,
-                                               unsigned long long* __restrict__ offmax) {
-  __shared__ double G[2][J2][J2 + 1];
-  __shared__ double Q[2][J2][J2 + 1];
-  __shared__ double red[4];
-  __shared__ int rotated[3];
-  const int pi = blockIdx.x, tid = threadIdx.x;
+// Pair eigensolve: sums the pair's Gram partials, records the normalised
+// off-diagonal, diagonalises G = Q diag Q^T in LDS and writes Q (or skip).
+__device__ void pair_eig(EigLds& sm, const double* __restrict__ Gpart, int nchunk, int pi,
+                         double tol, double* __restrict__ Qall, int* __restrict__ skip,
+                         unsigned long long* __restrict__ offmax, int nvI, int nvJ) {
+  auto& G = sm.G;
+  auto& Q = sm.Q;
+  auto& red = sm.red;
+  auto& rotated = sm.rotated;
+  const int tid = threadIdx.x;
   for (int e = tid; e < J2 * J2; e += 256) {
     double acc = 0.0;
     for (int c = 0; c < nchunk; ++c) acc += Gpart[((int64_t)pi * nchunk + c) * (J2 * J2) + e];
@@ -171,9 +201,9 @@ __global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart,
     return;
   }
   if (tid == 0) skip[pi] = 0;
-  // Parallel cyclic Jacobi, one barrier per round.  Thread t owns the 2x2
-  // block (row pair t/16, column pair t%16) of G' = J^T G J and of Q' = Q J,
-  // so each thread evaluates only the two rotations its block needs.
+  // Parallel cyclic Jacobi, two barriers per round: 16 threads compute the
+  // round's 16 rotations (FP64) into LDS, then thread t applies the 2x2 block
+  // (row pair t/16, column pair t%16) of G' = J^T G J and Q' = Q J.
   const double itol = tol;
   int cur = 0;
   // sweep flags rotate over 3 slots: slot (s+1)%3 is cleared during sweep s,
@@ -181,42 +211,48 @@ __global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart,
   if (tid == 0) rotated[0] = rotated[1] = rotated[2] = 0;
   __syncthreads();
   const int rp = tid >> 4, cp = tid & 15;
-  // At most two inner sweeps: the outer block iteration does not need each
-  // pair fully diagonalised, and late pairs are nearly diagonal already.
-  for (int sweep = 0; sweep < 2; ++sweep) {
+  // One inner sweep per outer step: the outer block iteration needs the
+  // same number of sweeps with one inner sweep as with full inner
+  // convergence (measured), at a fraction of the cost.
+  for (int sweep = 0; sweep < 1; ++sweep) {
     const int slot = sweep % 3;
     if (tid == 0) rotated[(sweep + 1) % 3] = 0;
     for (int r = 0; r < J2 - 1; ++r) {
       const int nxt = cur ^ 1;
-      // pair p occupies circle positions (p, 31 - p)
+      if (tid < J2 / 2) {
+        int x0 = tid == 0 ? 0 : 1 + (tid - 1 + r) % (J2 - 1);
+        int x1 = 1 + (J2 - 2 - tid + r) % (J2 - 1);
+        double a0, b0;
+        bool rx;
+        rot_coef(G[cur], x0, x1, itol, a0, b0, rx);  // a0 = c, b0 = -+s for x0
+        sm.cs[tid][0] = a0;
+        sm.cs[tid][1] = b0;
+        if (rx) rotated[slot] = 1;
+      }
+      __syncthreads();
       int x0 = rp == 0 ? 0 : 1 + (rp - 1 + r) % (J2 - 1);
       int x1 = 1 + (J2 - 2 - rp + r) % (J2 - 1);
       int y0 = cp == 0 ? 0 : 1 + (cp - 1 + r) % (J2 - 1);
       int y1 = 1 + (J2 - 2 - cp + r) % (J2 - 1);
-      double ax0, bx0, ax1, bx1, ay0, by0, ay1, by1;
-      bool rx, ry;
-      rot_coef(G[cur], x0, x1, itol, ax0, bx0, rx);
-      rot_coef(G[cur], x1, x0, itol, ax1, bx1, rx);
-      rot_coef(G[cur], y0, y1, itol, ay0, by0, ry);
-      rot_coef(G[cur], y1, y0, itol, ay1, by1, ry);
+      // cs holds (c, b) for the first index of each pair; the partner's b is -b
+      const double cx = sm.cs[rp][0], sx = sm.cs[rp][1];
+      const double cy = sm.cs[cp][0], sy = sm.cs[cp][1];
+      const double ax0 = cx, ax1 = cx;
+      const double bX0 = sx, bX1 = -sx, bY0 = sy, bY1 = -sy;
       const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
       const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
-      // rows first: R = J^T G restricted to rows {x0,x1}
-      const double r00 = ax0 * g00 + bx0 * g10, r01 = ax0 * g01 + bx0 * g11;
-      const double r10 = ax1 * g10 + bx1 * g00, r11 = ax1 * g11 + bx1 * g01;
-      // then columns: G' = R J restricted to columns {y0,y1}
-      G[nxt][x0][y0] = ay0 * r00 + by0 * r01;
-      G[nxt][x0][y1] = ay1 * r01 + by1 * r00;
-      G[nxt][x1][y0] = ay0 * r10 + by0 * r11;
-      G[nxt][x1][y1] = ay1 * r11 + by1 * r10;
-      // Q' = Q J: rows {x0,x1} (any row set), columns {y0,y1}
+      const double r00 = ax0 * g00 + bX0 * g10, r01 = ax0 * g01 + bX0 * g11;
+      const double r10 = ax1 * g10 + bX1 * g00, r11 = ax1 * g11 + bX1 * g01;
+      G[nxt][x0][y0] = cy * r00 + bY0 * r01;
+      G[nxt][x0][y1] = cy * r01 + bY1 * r00;
+      G[nxt][x1][y0] = cy * r10 + bY0 * r11;
+      G[nxt][x1][y1] = cy * r11 + bY1 * r10;
       const double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
       const double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
-      Q[nxt][x0][y0] = ay0 * q00 + by0 * q01;
-      Q[nxt][x0][y1] = ay1 * q01 + by1 * q00;
-      Q[nxt][x1][y0] = ay0 * q10 + by0 * q11;
-      Q[nxt][x1][y1] = ay1 * q11 + by1 * q10;
-      if (rp == cp && rx) rotated[slot] = 1;
+      Q[nxt][x0][y0] = cy * q00 + bY0 * q01;
+      Q[nxt][x0][y1] = cy * q01 + bY1 * q00;
+      Q[nxt][x1][y0] = cy * q10 + bY0 * q11;
+      Q[nxt][x1][y1] = cy * q11 + bY1 * q10;
       cur = nxt;
       __syncthreads();
     }
@@ -225,6 +261,18 @@ __global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart,
   }
   double* out = Qall + (int64_t)pi * (J2 * J2);
   for (int e = tid; e < J2 * J2; e += 256) out[e] = Q[cur][e / J2][e % J2];
+}
+
+__global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart, int nchunk,
+                                               int64_t nv, int s, int nblk, double tol,
+                                               double* __restrict__ Qall, int* __restrict__ skip,
+                                               unsigned long long* __restrict__ offmax) {
+  __shared__ EigLds sm;
+  int I, J;
+  pair_blocks(blockIdx.x, s, nblk, I, J);
+  pair_eig(sm, Gpart, nchunk, blockIdx.x, tol, Qall, skip, offmax,
+           (int)max<int64_t>(0, min<int64_t>(JB, nv - (int64_t)I * JB)),
+           (int)max<int64_t>(0, min<int64_t>(JB, nv - (int64_t)J * JB)));
 }
 
 // rows [ch*JCH, ...) of the stacked [Wv (L rows); Z (nv rows)] for pair pi
@@ -281,18 +329,21 @@ __global__ void __launch_bounds__(256) vec_norms(const double* __restrict__ Wv, 
   if (threadIdx.x == 0) sig[v] = sqrt(red[0] + red[1] + red[2] + red[3]);
 }
 
-// perm[rank] = j with rank = #{i : s_i > s_j or (s_i == s_j and i < j)}
+// perm[rank] = j with rank = #{i : s_i > s_j or (s_i == s_j and i < j)};
+// NaN ranks as -inf so that the ranks stay a permutation of 0..k-1.
+__device__ __forceinline__ double rank_key(double x) { return x == x ? x : -INFINITY; }
+
 __global__ void __launch_bounds__(1024) rank_sort(const double* __restrict__ sig, int64_t k,
                                                   int* __restrict__ perm, double* __restrict__ S) {
   for (int64_t j = threadIdx.x; j < k; j += 1024) {
-    double sj = sig[j];
+    const double sj = rank_key(sig[j]);
     int64_t rank = 0;
     for (int64_t i = 0; i < k; ++i) {
-      double si = sig[i];
+      const double si = rank_key(sig[i]);
       rank += (si > sj || (si == sj && i < j)) ? 1 : 0;
     }
     perm[rank] = (int)j;
-    S[rank] = sj;
+    S[rank] = sig[j];
   }
 }
 
@@ -344,6 +395,8 @@ __global__ void gather_vectors(const T* __restrict__ src, Modes rows, Modes cols
   }
 }
 
+__global__ void reset_u64(unsigned long long* p) { *p = 0ull; }
+
 __global__ void init_identity(double* __restrict__ Z, int64_t k) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= k * k) return;
@@ -352,7 +405,7 @@ __global__ void init_identity(double* __restrict__ Z, int64_t k) {
 
 struct SvdWs {
   double *Wv, *Z, *Gpart, *Q, *sig, *S, *R, *QZ;
-  int *skip, *perm;
+  int *skip, *perm, *counter;
   unsigned long long* off;
   QrWs qr;
   int precond;
@@ -378,6 +431,7 @@ static size_t svd_layout(int64_t m, int64_t n, char* base, size_t cap, SvdWs* ou
   w.skip = a.take<int>((size_t)npairs);
   w.perm = a.take<int>((size_t)k);
   w.off = a.take<unsigned long long>(4);
+  w.counter = a.take<int>((size_t)npairs);
   if (w.precond) {
     w.R = a.take<double>((size_t)(k * k));
     w.QZ = a.take<double>((size_t)(L * k));
@@ -396,37 +450,75 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
   int nblk = (int)ceil_div(std::max<int64_t>(k, 1), JB);
   if (nblk % 2) nblk += 1;
   const int npairs = nblk / 2;
-  const int nchW = (int)ceil_div(L, JCH), nchZ = (int)ceil_div(k, JCH);
-  hipLaunchKernelGGL(init_identity, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, Z, k);
+  // Z == nullptr: the rotations are not accumulated (eigensolver use)
+  const int nchW = (int)ceil_div(L, JCH), nchZ = Z ? (int)ceil_div(k, JCH) : 0;
+  const int64_t kz = Z ? k : 0;
+  if (Z)
+    hipLaunchKernelGGL(init_identity, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, Z, k);
   TN_CHECK_LAUNCH();
+  TN_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * npairs, st));
   int sweep = 0;
   *converged = false;
   unsigned long long host_off = 0;
-  for (; sweep < max_sweeps; ++sweep) {
-    TN_HIP(hipMemsetAsync(w.off, 0, sizeof(unsigned long long), st));
-    // algorithmic work per step: Gram 2 L (2b)^2 per pair, apply 2 (L + k) (2b)^2 per pair
-    const double gram_fl = 2.0 * (double)L * J2 * J2 * npairs;
-    const double apply_fl = 2.0 * (double)(L + k) * J2 * J2 * npairs;
+  // algorithmic work per step: Gram 2 L (2b)^2 per pair, apply 2 (L + k) (2b)^2 per pair
+  const double gram_fl = 2.0 * (double)L * J2 * J2 * npairs;
+  const double apply_fl = 2.0 * (double)(L + kz) * J2 * J2 * npairs;
+  auto enqueue_sweep = [&]() -> int {
+    hipLaunchKernelGGL(reset_u64, dim3(1), dim3(1), 0, st, w.off);
     for (int s = 0; s < nblk - 1; ++s) {
       {
+        // Gram partials + (last block per pair) the pair eigensolve
         ProfScope p(PROF_JAC_GRAM, st, gram_fl, 8.0 * (double)L * J2 * npairs);
         hipLaunchKernelGGL(jac_gram, dim3(npairs, nchW), dim3(256), 0, st, Wv, L, k, s, nblk,
-                           nchW, w.Gpart);
+                           nchW, w.Gpart, w.counter, tol, w.Q, w.skip, w.off);
       }
       {
-        ProfScope p(PROF_JAC_EIG, st, 0.0, 8.0 * (double)nchW * J2 * J2 * npairs);
-        hipLaunchKernelGGL(jac_eig, dim3(npairs), dim3(256), 0, st, w.Gpart, nchW, k, s, nblk,
-                           tol, w.Q, w.skip, w.off);
-      }
-      {
-        ProfScope p(PROF_JAC_APPLY, st, apply_fl, 16.0 * (double)(L + k) * J2 * npairs);
+        ProfScope p(PROF_JAC_APPLY, st, apply_fl, 16.0 * (double)(L + kz) * J2 * npairs);
         hipLaunchKernelGGL(jac_apply, dim3(npairs, nchW + nchZ), dim3(256), 0, st, Wv, L, Z, k,
                            s, nblk, nchW, w.Q, w.skip);
       }
-      TN_CHECK_LAUNCH();
     }
-    TN_HIP(hipMemcpyAsync(&host_off, w.off, sizeof(host_off), hipMemcpyDeviceToHost, st));
-    TN_HIP(hipStreamSynchronize(st));
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+  };
+  // A sweep is the same launch sequence every time: capture it once as a HIP
+  // graph and replay it (long sweeps only; not while launch timing is on).
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (nblk - 1 >= 16 && !prof_on()) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      int rc = enqueue_sweep();
+      hipError_t e = hipStreamEndCapture(st, &graph);
+      if (rc != TN_OK || e != hipSuccess ||
+          hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        graph = nullptr;
+        exec = nullptr;
+        (void)hipGetLastError();
+      }
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  int rc_loop = TN_OK;
+  for (; sweep < max_sweeps; ++sweep) {
+    if (exec) {
+      if (hipGraphLaunch(exec, st) != hipSuccess) {
+        set_error(TN_ERR_HIP, "tn_svd: graph launch failed");
+        rc_loop = TN_ERR_HIP;
+        break;
+      }
+    } else {
+      rc_loop = enqueue_sweep();
+      if (rc_loop != TN_OK) break;
+    }
+    if (hipMemcpyAsync(&host_off, w.off, sizeof(host_off), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      set_error(TN_ERR_HIP, "tn_svd: convergence readback failed");
+      rc_loop = TN_ERR_HIP;
+      break;
+    }
     double off;
     memcpy(&off, &host_off, sizeof(off));
     if (off <= tol) {
@@ -435,6 +527,9 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
       break;
     }
   }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc_loop != TN_OK) return rc_loop;
   if (sweeps_out) *sweeps_out = sweep;
   return TN_OK;
 }
@@ -520,6 +615,236 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
   TN_CHECK_LAUNCH();
   if (!converged) {
     set_error(TN_ERR_NOCONV, "tn_svd: Jacobi did not converge in %d sweeps", max_sweeps);
+    return TN_ERR_NOCONV;
+  }
+  return TN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Eigendecomposition of a symmetric positive semi-definite matrix,
+// K = U diag(lam) U^T, for the truncation step of the structured
+// apply+compress sweep, where only the bond's Gram matrix K = C^T E C is
+// formed (the reference takes the SVD of C's left-isometric completion,
+// onedim_core.py:317 -> tensor.py:915; K has the same left vectors and
+// lam = sigma^2).
+//
+// Method:
+//   1. P orders K by descending diagonal (a static pivot); shift
+//      delta = 8 sqrt(n) eps_in tr(K), eps_in the input's unit roundoff;
+//   2. P K P^T + delta I = L L^T, blocked right-looking Cholesky (64-column
+//      blocks: the diagonal block in LDS, the panel by a GEMM with the
+//      inverted diagonal block, the trailing update by an MFMA GEMM; a
+//      column whose pivot falls below delta/2 is dropped);
+//   3. one-sided block Jacobi on the columns of L (no rotation accumulation):
+//      L Z = U' Sigma, hence P K P^T + delta I = U' Sigma^2 U'^T, so
+//      U = P^T U' and lam = sigma^2 - delta.
+// Jacobi on the Cholesky factor works on a matrix with the singular values
+// sqrt(lam); Jacobi on K itself would work on K^2, whose condition number
+// reaches the FP64 floor for truncated-MPS Gram matrices and needs about
+// three times as many sweeps.  The shift leaves the eigenvectors unchanged.
+// ---------------------------------------------------------------------------
+namespace tn {
+
+constexpr int CB = 64;  // Cholesky block
+
+// diag[j] = A[j][j]; delta = 8 sqrt(n) eps_in sum_j |A[j][j]|
+__global__ void __launch_bounds__(256) psd_diag(const double* __restrict__ A, int64_t n,
+                                                double eps_in, double* __restrict__ diag,
+                                                double* __restrict__ delta) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 256) {
+    const double v = A[j * n + j];
+    diag[j] = v;
+    acc += fabs(v);
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    *delta = 8.0 * sqrt((double)n) * eps_in * (red[0] + red[1] + red[2] + red[3]);
+}
+
+// B[c][r] = sym(A)[p[c]][p[r]] + delta [r == c]   (column-major, ld n);
+// the symmetric part, since a K formed by rounded products is symmetric only
+// to rounding
+__global__ void psd_permute(const double* __restrict__ A, int64_t n, const int* __restrict__ p,
+                            const double* __restrict__ delta, double* __restrict__ B) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t c = idx / n, r = idx - c * n;
+  const int64_t pc = p[c], pr = p[r];
+  double v = 0.5 * (A[pc * n + pr] + A[pr * n + pc]);
+  if (r == c) v += *delta;
+  B[idx] = v;
+}
+
+// Factor the w x w diagonal block at (j0, j0) of B (column-major, ld n):
+// writes L11 (lower, zeros above) into Lw at the same position and
+// T = L11^{-T} (row-major CB x CB) into Tinv.  One wavefront: thread t owns
+// row t of the block.
+__global__ void __launch_bounds__(64) chol_diag(const double* __restrict__ B, int64_t n,
+                                                int64_t j0, int w,
+                                                const double* __restrict__ delta,
+                                                double* __restrict__ Lw,
+                                                double* __restrict__ Tinv) {
+  __shared__ double a[CB][CB + 1];
+  __shared__ double x[CB][CB + 1];
+  const int t = threadIdx.x;
+  const double dl = *delta;
+  for (int c = 0; c < w; ++c) a[t][c] = t < w ? B[(j0 + c) * n + j0 + t] : 0.0;
+  __syncthreads();
+  for (int c = 0; c < w; ++c) {
+    // With the shift every exact pivot is >= delta; one below delta/2 means
+    // the input is indefinite beyond its rounding allowance: the column is
+    // dropped (zero) rather than divided by a tiny pivot.
+    const double d = a[c][c];
+    const double s = d >= 0.5 * dl && d > 0.0 ? sqrt(d) : 0.0;
+    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    __syncthreads();  // a[c][c] read by every thread before it is replaced
+    if (t == c) a[c][c] = s;
+    if (t > c && t < w) a[t][c] *= inv;
+    __syncthreads();
+    if (t > c && t < w) {
+      const double l = a[t][c];
+      for (int c2 = c + 1; c2 <= t; ++c2) a[t][c2] -= l * a[c2][c];
+    }
+    __syncthreads();
+  }
+  // X = L11^{-1}, column t by forward substitution
+  if (t < w) {
+    for (int i = 0; i < w; ++i) {
+      double v = 0.0;
+      if (i >= t) {
+        double acc = i == t ? 1.0 : 0.0;
+        for (int k = t; k < i; ++k) acc -= a[i][k] * x[k][t];
+        const double dd = a[i][i];
+        v = dd > 0.0 ? acc / dd : 0.0;
+      }
+      x[i][t] = v;
+    }
+  }
+  __syncthreads();
+  if (t < w) {
+    for (int c = 0; c < w; ++c) Lw[(j0 + c) * n + j0 + t] = t >= c ? a[t][c] : 0.0;
+    for (int c = 0; c < w; ++c) Tinv[t * CB + c] = x[c][t];  // T[t][c] = X[c][t]
+  }
+}
+
+// U[p[i]][j] = Wv[perm[j]][i] / S[j];  lam[j] = S[j]^2 - delta
+__global__ void eig_out_kernel(const double* __restrict__ Wv, int64_t n, const int* __restrict__ p,
+                               const int* __restrict__ perm, const double* __restrict__ S,
+                               const double* __restrict__ delta, double* __restrict__ U,
+                               double* __restrict__ lam) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t i = idx / n, j = idx - i * n;
+  const double s = S[j];
+  U[(int64_t)p[i] * n + j] = s > 0.0 ? Wv[(int64_t)perm[j] * n + i] / s : 0.0;
+  if (i == 0) lam[j] = s * s - *delta;
+}
+
+struct EigWs {
+  SvdWs jw;  // Jacobi buffers (Wv holds L)
+  double *B, *Tinv, *diag, *Sd, *delta;
+  int* p;
+};
+
+static size_t eig_layout(int64_t n, char* base, size_t cap, EigWs* out) {
+  EigWs e{};
+  const int64_t nb0 = ceil_div(std::max<int64_t>(n, 1), JB);
+  const int nblk = (int)(nb0 + nb0 % 2);
+  const int npairs = nblk / 2;
+  Arena a{base, cap, 0};
+  e.jw.Wv = a.take<double>((size_t)(n * n));
+  e.B = a.take<double>((size_t)(n * n));
+  e.Tinv = a.take<double>((size_t)CB * CB);
+  e.diag = a.take<double>((size_t)n);
+  e.Sd = a.take<double>((size_t)n);
+  e.delta = a.take<double>(4);
+  e.p = a.take<int>((size_t)n);
+  e.jw.Gpart = a.take<double>((size_t)npairs * ceil_div(n, JCH) * J2 * J2);
+  e.jw.Q = a.take<double>((size_t)npairs * J2 * J2);
+  e.jw.sig = a.take<double>((size_t)n);
+  e.jw.S = a.take<double>((size_t)n);
+  e.jw.skip = a.take<int>((size_t)npairs);
+  e.jw.perm = a.take<int>((size_t)n);
+  e.jw.off = a.take<unsigned long long>(4);
+  e.jw.counter = a.take<int>((size_t)npairs);
+  if (out) *out = e;
+  return a.used;
+}
+
+}  // namespace tn
+
+extern "C" size_t tn_eigh_psd_workspace(int64_t n) {
+  if (n <= 0) return 256;
+  return eig_layout(n, nullptr, 0, nullptr) + 256;
+}
+
+extern "C" int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes* k_cols, int64_t n,
+                           double* U, double* lam, int dtype, double tol, int max_sweeps,
+                           int* sweeps_out, void* ws, size_t ws_bytes, void* stream) {
+  TN_REQUIRE(n >= 0, "tn_eigh_psd: negative size");
+  TN_REQUIRE(dtype == TN_F64 || dtype == TN_F32, "tn_eigh_psd: unsupported dtype %d", dtype);
+  Modes rows = to_modes(k_rows), cols = to_modes(k_cols);
+  TN_REQUIRE(modes_size(rows) == n && modes_size(cols) == n,
+             "tn_eigh_psd: modes do not match n x n");
+  if (sweeps_out) *sweeps_out = 0;
+  if (n == 0) return TN_OK;
+  TN_REQUIRE(K && U && lam, "tn_eigh_psd: null pointer");
+  TN_REQUIRE(ws && ws_bytes >= tn_eigh_psd_workspace(n), "tn_eigh_psd: workspace too small");
+  TN_REQUIRE(n <= (int64_t)1 << 15, "tn_eigh_psd: too large");
+  if (max_sweeps <= 0) max_sweeps = 40;
+  hipStream_t st = as_stream(stream);
+  EigWs e{};
+  eig_layout(n, (char*)ws, ws_bytes, &e);
+  double* Wv = e.jw.Wv;
+  // 1. gather K column-major into Wv (scratch), diagonal, shift, static pivot
+  unsigned gb = (unsigned)std::min<int64_t>(ceil_div(n * n, 256), 65536);
+  if (dtype == TN_F64)
+    hipLaunchKernelGGL(gather_vectors<double>, dim3(gb), dim3(256), 0, st, (const double*)K, rows,
+                       cols, n, n, 1, Wv);
+  else
+    hipLaunchKernelGGL(gather_vectors<float>, dim3(gb), dim3(256), 0, st, (const float*)K, rows,
+                       cols, n, n, 1, Wv);
+  // shift allowance from the input's precision (K formed in FP32 is
+  // indefinite at ~eps_32 |K|)
+  const double eps_in = dtype == TN_F64 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+  hipLaunchKernelGGL(psd_diag, dim3(1), dim3(256), 0, st, Wv, n, eps_in, e.diag, e.delta);
+  hipLaunchKernelGGL(rank_sort, dim3(1), dim3(1024), 0, st, e.diag, n, e.p, e.Sd);
+  const unsigned nb = (unsigned)ceil_div(n * n, 256);
+  hipLaunchKernelGGL(psd_permute, dim3(nb), dim3(256), 0, st, Wv, n, e.p, e.delta, e.B);
+  TN_CHECK_LAUNCH();
+  TN_HIP(hipMemsetAsync(Wv, 0, sizeof(double) * (size_t)(n * n), st));
+  // 2. blocked Cholesky: B (working copy) -> L in Wv
+  for (int64_t j0 = 0; j0 < n; j0 += CB) {
+    const int w = (int)std::min<int64_t>(CB, n - j0);
+    hipLaunchKernelGGL(chol_diag, dim3(1), dim3(64), 0, st, e.B, n, j0, w, e.delta, Wv, e.Tinv);
+    TN_CHECK_LAUNCH();
+    const int64_t m2 = n - j0 - w;
+    if (m2 <= 0) break;
+    // L21 = A21 T   (m2 x w)
+    TN_TRY(contract_launch(e.B + j0 * n + j0 + w, mk({m2}, {1}), mk({w}, {n}), e.Tinv,
+                           mk({w}, {CB}), mk({w}, {1}), Wv + j0 * n + j0 + w, mk({m2}, {1}),
+                           mk({w}, {n}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, nullptr, 0, st));
+    // A22 -= L21 L21^T
+    const double* L21 = Wv + j0 * n + j0 + w;
+    TN_TRY(contract_launch(L21, mk({m2}, {1}), mk({w}, {n}), L21, mk({w}, {n}), mk({m2}, {1}),
+                           e.B + (j0 + w) * n + j0 + w, mk({m2}, {1}), mk({m2}, {n}), TN_F64,
+                           TN_PREC_NATIVE, -1.0, 1.0, nullptr, 0, st));
+  }
+  // 3. one-sided Jacobi on the columns of L (length n)
+  if (tol <= 0.0) tol = std::max(1e-15, (double)n * 2.220446049250313e-16);
+  bool converged = false;
+  TN_TRY(jacobi_run(Wv, n, nullptr, n, e.jw, tol, max_sweeps, sweeps_out, &converged, st));
+  hipLaunchKernelGGL(vec_norms, dim3((unsigned)n), dim3(256), 0, st, Wv, n, e.jw.sig);
+  hipLaunchKernelGGL(rank_sort, dim3(1), dim3(1024), 0, st, e.jw.sig, n, e.jw.perm, e.jw.S);
+  hipLaunchKernelGGL(eig_out_kernel, dim3(nb), dim3(256), 0, st, Wv, n, e.p, e.jw.perm, e.jw.S,
+                     e.delta, U, lam);
+  TN_CHECK_LAUNCH();
+  if (!converged) {
+    set_error(TN_ERR_NOCONV, "tn_eigh_psd: Jacobi did not converge in %d sweeps", max_sweeps);
     return TN_ERR_NOCONV;
   }
   return TN_OK;

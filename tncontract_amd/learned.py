@@ -143,15 +143,15 @@ def rsvd(A, rank, oversample=10, power_iters=2, seed=0, precision="native"):
     residual ||A - U S Vt||_F / ||A||_F."""
     m, n = A.shape
     p = min(rank + oversample, m, n)
-    A64 = L.convert(A, torch.float64)
+    A64 = L.convert(A, torch.float64)          # FP32 data is exact in FP64
     con = lambda x, ax1, y, ax2: tsr._contract_data(x, ax1, y, ax2, precision)  # noqa: E731
-    Om = randn((n, p), dtype=A.dtype, seed=seed)
-    Y = con(A, [1], Om, [0])                   # m x p
+    Om = randn((n, p), dtype=torch.float64, seed=seed)
+    Y = con(A64, [1], Om, [0])                 # m x p
     for _ in range(power_iters):
-        Q = L.convert(_orth(Y), A.dtype)
-        Z = con(A, [0], Q, [0])                # n x p  (A^T Q)
-        Qz = L.convert(_orth(Z), A.dtype)
-        Y = con(A, [1], Qz, [0])               # m x p
+        Q = _orth(Y)
+        Z = con(A64, [0], Q, [0])              # n x p  (A^T Q)
+        Qz = _orth(Z)
+        Y = con(A64, [1], Qz, [0])             # m x p
     Q = _orth(Y)                               # m x p, FP64
     Bm = tsr._contract_data(Q, [0], A64, [0])  # p x n = Q^T A (FP64)
     ub, s, vt = tsr._svd_device(Bm, 1)

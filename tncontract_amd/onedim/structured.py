@@ -21,7 +21,8 @@ bond then depends on the left part through its Gram matrix alone.  So:
    contraction), each normalised on the device.
 2. Right-to-left truncation carrying P_{i+1} (right bond of the fused site x
    kept rank):  C_i = A_i P_{i+1};  K_i = C_i^T E_i C_i  (d r x d r, FP64);
-   K_i = Y diag(sigma^2) Y^T (one-sided Jacobi, FP64 rotations); keep the
+   K_i = Y diag(sigma^2) Y^T (Cholesky-preconditioned one-sided Jacobi, FP64
+   rotations: tn_eigh_psd); keep the
    reference's rule (sigma/sigma_0 > threshold, at most chi, at most the QR
    bond dimension the reference's left sweep would have produced);  site i =
    Y_kept (right-isometric, labels [physout, right, left]);  P_i = C_i Y_kept.
@@ -111,8 +112,10 @@ def apply_mpo_compress(mps, mpo, chi=None, threshold=1e-15, precision="fp64", st
         Kt = con(C, [0, 2], F, [0, 1])                    # (k', t', k, t)
         r = C.shape[1]
         n = d[i] * r
-        Km = _cdt(Kt, torch.float64).permute(1, 0, 3, 2)  # rows (t', k'), cols (t, k)
-        u, lam, _ = tsr._svd_device(Km, 2) if eig_tol == 0.0 else _svd_tol(Km, eig_tol)
+        # rows (t', k'), cols (t, k); an FP32 K goes in as FP32 so that the
+        # eigensolver's shift covers its rounding
+        Km = Kt.permute(1, 0, 3, 2)
+        u, lam = tsr._eigh_psd_device(Km, 2, eig_tol)
         lam_h = lam.cpu().numpy()
         avail = min(n, qb[i])
         sig = np.sqrt(np.maximum(lam_h[:avail], 0.0))
@@ -138,17 +141,3 @@ def apply_mpo_compress(mps, mpo, chi=None, threshold=1e-15, precision="fp64", st
     return core.MatrixProductState._from_owned(sites, mps.left_label, mps.right_label,
                                                mpo.physout_label)
 
-
-def _svd_tol(Km, tol):
-    import ctypes
-    rows, cols, m, n = tsr._matrix_modes(Km, 2)
-    k = min(m, n)
-    u = torch.empty((m, k), dtype=Km.dtype, device=Km.device)
-    s = torch.empty((k,), dtype=torch.float64, device=Km.device)
-    vt = torch.empty((k, n), dtype=Km.dtype, device=Km.device)
-    ws = L.workspace(L.lib().tn_svd_workspace(m, n, L.dtype_code(Km)))
-    sw = ctypes.c_int(0)
-    L.check(L.lib().tn_svd(L.ptr(Km), ctypes.byref(rows), ctypes.byref(cols), m, n, L.ptr(u),
-                           L.ptr(s), L.ptr(vt), L.dtype_code(Km), float(tol), 0, ctypes.byref(sw),
-                           L.ptr(ws), ws.numel(), L.stream_ptr()))
-    return u, s, vt

@@ -557,6 +557,27 @@ def _svd_device(d, nrow):
     return u, s, vt
 
 
+def _eigh_psd_device(d, nrow, tol=0.0):
+    """Eigendecomposition of the symmetric positive semi-definite (rows | cols)
+    matrix view of d: returns (u, lam) with u: n x n float64 (column j =
+    eigenvector j) and lam: n float64, descending."""
+    rows, cols, m, n = _matrix_modes(d, nrow)
+    if m != n:
+        raise ValueError("eigh_psd needs a square matrix, got %d x %d" % (m, n))
+    u = torch.empty((n, n), dtype=torch.float64, device=d.device)
+    lam = torch.empty((n,), dtype=torch.float64, device=d.device)
+    if n == 0:
+        return u, lam
+    lib = L.lib()
+    ws = L.workspace(lib.tn_eigh_psd_workspace(n))
+    sweeps = ctypes.c_int(0)
+    rc = lib.tn_eigh_psd(L.ptr(d), ctypes.byref(rows), ctypes.byref(cols), n, L.ptr(u),
+                         L.ptr(lam), L.dtype_code(d), float(tol), 0, ctypes.byref(sweeps),
+                         L.ptr(ws), ws.numel(), L.stream_ptr())
+    L.check(rc)
+    return u, lam
+
+
 def _absorb(U, S, V, how):
     # tensor.py:945-956 (the reference hard-codes "svd_in"/"svd_out" here)
     if how == "left":
