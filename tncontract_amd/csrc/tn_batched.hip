@@ -17,6 +17,7 @@
 //  * per-job sort, rank selection (sigma/sigma_0 > threshold, <= chi) and
 //    output of U, S, V^T.
 #include "tn_internal.h"
+#include "tn_jacobi.h"
 #include "tn_prof.h"
 
 #include <algorithm>
@@ -108,6 +109,7 @@ __global__ void __launch_bounds__(256) gemm_grouped_f32(const GJob* __restrict__
 
 // ---------------------------------------------------------------- batched Jacobi
 constexpr int BJB = 16, BJ2 = 32, BCH = 256;
+static_assert(BJB == JB && BJ2 == J2, "batched pairs use the shared pair layout");
 
 struct BJob {
   int64_t wv;     // offset of the job's vectors (k vectors of length L)
@@ -164,23 +166,6 @@ __global__ void __launch_bounds__(256) bjac_gram(double* __restrict__ Wv, const 
   }
 }
 
-__device__ __forceinline__ void brot(const double (*G)[BJ2 + 1], int x, int px, double tol,
-                                     double& a, double& b, bool& rot) {
-  int l = min(x, px), h = max(x, px);
-  double gpq = G[l][h], gpp = G[l][l], gqq = G[h][h];
-  double c = 1.0, sn = 0.0;
-  rot = false;
-  if (gpq != 0.0 && fabs(gpq) > tol * sqrt(fmax(gpp * gqq, 0.0))) {
-    double th = (gqq - gpp) / (2.0 * gpq);
-    double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
-    c = 1.0 / sqrt(1.0 + t * t);
-    sn = t * c;
-    rot = true;
-  }
-  a = c;
-  b = x == l ? -sn : sn;
-}
-
 // eig items are (job, pair) in job-major order; Gpart items for (job, pair,
 // chunk) start at gram_base + pair * nch.
 __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart,
@@ -190,11 +175,7 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
                                                 const int* __restrict__ done,
                                                 double* __restrict__ Qall, int* __restrict__ skip,
                                                 unsigned long long* __restrict__ offmax) {
-  __shared__ double G[2][BJ2][BJ2 + 1];
-  __shared__ double Q[2][BJ2][BJ2 + 1];
-  __shared__ double red[4];
-  __shared__ int rotated[1];
-  __shared__ double cs[BJ2 / 2][2];
+  __shared__ EigLds sm;
   const int item = blockIdx.x, tid = threadIdx.x;
   const int j = e_job[item];
   if (done[j]) return;
@@ -204,71 +185,20 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
     double acc = 0.0;
     for (int c = 0; c < J.nch; ++c)
       acc += Gpart[((int64_t)J.gram_base + (int64_t)pi * J.nch + c) * (BJ2 * BJ2) + e];
-    G[0][e / BJ2][e % BJ2] = acc;
-    Q[0][e / BJ2][e % BJ2] = (e / BJ2 == e % BJ2) ? 1.0 : 0.0;
+    sm.G[0][e / BJ2][e % BJ2] = acc;
+    sm.Q[0][e / BJ2][e % BJ2] = (e / BJ2 == e % BJ2) ? 1.0 : 0.0;
   }
   __syncthreads();
-  double off = 0.0;
-  for (int e = tid; e < BJ2 * BJ2; e += 256) {
-    int p = e / BJ2, q = e % BJ2;
-    if (p < q) {
-      double d = G[0][p][p] * G[0][q][q];
-      if (d > 0.0) off = fmax(off, fabs(G[0][p][q]) / sqrt(d));
-    }
-  }
-  off = wave_max(off);
-  if ((tid & 63) == 0) red[tid >> 6] = off;
-  __syncthreads();
-  off = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-  if (tid == 0) atomicMax(offmax + j, (unsigned long long)__double_as_longlong(off));
+  const double off = eig_off(sm);
+  if (tid == 0) atomicMax(offmax + j, dbits(off));
   if (off <= J.tol) {
     if (tid == 0) skip[item] = 1;
     return;
   }
   if (tid == 0) skip[item] = 0;
-  int cur = 0;
-  if (tid == 0) rotated[0] = 0;
-  __syncthreads();
-  const int rp = tid >> 4, cp = tid & 15;
-  // one inner sweep, two barriers per round: 16 threads compute the round's
-  // rotations, then every thread applies its 2x2 block of G' and Q'
-  for (int r = 0; r < BJ2 - 1; ++r) {
-    const int nxt = cur ^ 1;
-    if (tid < BJ2 / 2) {
-      int x0 = tid == 0 ? 0 : 1 + (tid - 1 + r) % (BJ2 - 1);
-      int x1 = 1 + (BJ2 - 2 - tid + r) % (BJ2 - 1);
-      double a0, b0;
-      bool rx;
-      brot(G[cur], x0, x1, J.tol, a0, b0, rx);
-      cs[tid][0] = a0;
-      cs[tid][1] = b0;
-    }
-    __syncthreads();
-    int x0 = rp == 0 ? 0 : 1 + (rp - 1 + r) % (BJ2 - 1);
-    int x1 = 1 + (BJ2 - 2 - rp + r) % (BJ2 - 1);
-    int y0 = cp == 0 ? 0 : 1 + (cp - 1 + r) % (BJ2 - 1);
-    int y1 = 1 + (BJ2 - 2 - cp + r) % (BJ2 - 1);
-    const double cx = cs[rp][0], bX0 = cs[rp][1], bX1 = -cs[rp][1];
-    const double cy = cs[cp][0], bY0 = cs[cp][1], bY1 = -cs[cp][1];
-    const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
-    const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
-    const double r00 = cx * g00 + bX0 * g10, r01 = cx * g01 + bX0 * g11;
-    const double r10 = cx * g10 + bX1 * g00, r11 = cx * g11 + bX1 * g01;
-    G[nxt][x0][y0] = cy * r00 + bY0 * r01;
-    G[nxt][x0][y1] = cy * r01 + bY1 * r00;
-    G[nxt][x1][y0] = cy * r10 + bY0 * r11;
-    G[nxt][x1][y1] = cy * r11 + bY1 * r10;
-    const double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
-    const double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
-    Q[nxt][x0][y0] = cy * q00 + bY0 * q01;
-    Q[nxt][x0][y1] = cy * q01 + bY1 * q00;
-    Q[nxt][x1][y0] = cy * q10 + bY0 * q11;
-    Q[nxt][x1][y1] = cy * q11 + bY1 * q10;
-    cur = nxt;
-    __syncthreads();
-  }
+  const int cur = eig_rotate(sm, J.tol);
   double* out = Qall + (int64_t)item * (BJ2 * BJ2);
-  for (int e = tid; e < BJ2 * BJ2; e += 256) out[e] = Q[cur][e / BJ2][e % BJ2];
+  for (int e = tid; e < BJ2 * BJ2; e += 256) out[e] = sm.Q[cur][e / BJ2][e % BJ2];
 }
 
 // apply items are (job, pair, chunk) over the L + k rows of [W; Z]
@@ -280,7 +210,7 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
                                                   const int* __restrict__ done, int g,
                                                   const double* __restrict__ Qall,
                                                   const int* __restrict__ skip) {
-  __shared__ double Q[BJ2][BJ2];
+  __shared__ double Q[BJ2][BJ2 + 1];
   const int item = blockIdx.x;
   const int j = a_job[item];
   if (done[j]) return;
@@ -292,29 +222,12 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
   const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
   for (int e = threadIdx.x; e < BJ2 * BJ2; e += 256) Q[e / BJ2][e % BJ2] = Qall[(int64_t)eidx * BJ2 * BJ2 + e];
   __syncthreads();
-  double* base;
-  int64_t ld, r;
   if (ch < J.nch) {
-    base = Wv + J.wv; ld = J.L; r = (int64_t)ch * BCH + threadIdx.x;
-    if (r >= J.L) return;
+    const int64_t r = (int64_t)ch * BCH;
+    apply_mfma(Wv + J.wv, J.L, J.k, r, min((int64_t)J.L, r + BCH), I, K2, Q);
   } else {
-    base = Zb + J.z; ld = J.k; r = (int64_t)(ch - J.nch) * BCH + threadIdx.x;
-    if (r >= J.k) return;
-  }
-  double x[BJ2];
-#pragma unroll
-  for (int c = 0; c < BJ2; ++c) {
-    int64_t v = c < BJB ? (int64_t)I * BJB + c : (int64_t)K2 * BJB + (c - BJB);
-    x[c] = v < J.k ? base[v * ld + r] : 0.0;
-  }
-#pragma unroll
-  for (int c = 0; c < BJ2; ++c) {
-    int64_t v = c < BJB ? (int64_t)I * BJB + c : (int64_t)K2 * BJB + (c - BJB);
-    if (v >= J.k) continue;
-    double acc = 0.0;
-#pragma unroll
-    for (int i = 0; i < BJ2; ++i) acc = fma(x[i], Q[i][c], acc);
-    base[v * ld + r] = acc;
+    const int64_t r = (int64_t)(ch - J.nch) * BCH;
+    apply_mfma(Zb + J.z, J.k, J.k, r, min((int64_t)J.k, r + BCH), I, K2, Q);
   }
 }
 

@@ -19,7 +19,9 @@
 // off-diagonal of a whole sweep is <= tol.  Singular values are the vector
 // norms; they are sorted descending on the device.
 #include "tn_internal.h"
+#include "tn_jacobi.h"
 #include "tn_prof.h"
+
 
 #include <cmath>
 #include <cstring>
@@ -27,50 +29,15 @@
 
 namespace tn {
 
-constexpr int JB = 16;       // vectors per block
-constexpr int J2 = 2 * JB;   // pair width
 constexpr int JCH = 256;     // rows per chunk (jac_gram / jac_apply)
 
-__device__ __forceinline__ int rr_block(int pos, int step, int nblk) {
-  return pos == 0 ? 0 : 1 + (pos - 1 + step) % (nblk - 1);
-}
-
-// pair pi at step s -> blocks (I, J)
-__device__ __forceinline__ void pair_blocks(int pi, int s, int nblk, int& I, int& J) {
-  I = rr_block(pi, s, nblk);
-  J = rr_block(nblk - 1 - pi, s, nblk);
-}
-
-__device__ __forceinline__ int64_t pair_vec(int c, int I, int J) {
-  return c < JB ? (int64_t)I * JB + c : (int64_t)J * JB + (c - JB);
-}
-
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-struct EigLds {
-  double G[2][J2][J2 + 1];
-  double Q[2][J2][J2 + 1];
-  double cs[J2 / 2][2];
-  double red[4];
-  int rotated[3];
-};
-__device__ void pair_eig(EigLds& sm, const double* __restrict__ Gpart, int nchunk, int pi,
-                         double tol, double* __restrict__ Qall, int* __restrict__ skip,
-                         unsigned long long* __restrict__ offmax, int nvI, int nvJ);
-
-__global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, int64_t L,
-                                                int64_t nv, int s, int nblk, int nchunk,
-                                                double* __restrict__ Gpart, int* counter,
-                                                double tol, double* __restrict__ Qall,
-                                                int* __restrict__ skip,
-                                                unsigned long long* __restrict__ offmax) {
-  __shared__ double Ps[2][64][J2 + 1];
-  const int pi = blockIdx.x, ch = blockIdx.y;
-  int I, J;
-  pair_blocks(pi, s, nblk, I, J);
+// Partial Gram P^T P of the pair's 32 vectors over rows [r0, r1) into out
+// (32 x 32, row stride ostride; global or LDS), v_mfma_f64_16x16x4 from a
+// double-buffered LDS tile.  256 threads.
+__device__ void gram_partial(double (*Ps)[64][J2 + 1], const double* __restrict__ Wv, int64_t L,
+                             int64_t nv, int I, int J, int64_t r0, int64_t r1,
+                             double* __restrict__ out, int ostride = J2) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int64_t r0 = (int64_t)ch * JCH;
-  const int64_t r1 = min(L, r0 + JCH);
   const int ti = wave >> 1, tj = wave & 1;
   f64x4 acc = {0, 0, 0, 0};
   double reg[8];
@@ -109,170 +76,78 @@ __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, i
     if (sb + 1 < nsub) store(buf ^ 1);
     __syncthreads();
   }
-  double* out = Gpart + ((int64_t)pi * nchunk + ch) * (J2 * J2);
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     int r = 16 * ti + (lane >> 4) + 4 * e, c = 16 * tj + (lane & 15);
-    out[r * J2 + c] = acc[e];
+    out[r * ostride + c] = acc[e];
   }
-  if (counter == nullptr) return;
-  // Last-arriving chunk block of the pair runs the pair eigensolve: release
-  // the partial device-wide, count arrivals, acquire before reading partials.
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) last = (atomicAdd(counter + pi, 1) == nchunk - 1) ? 1 : 0;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (tid == 0) counter[pi] = 0;  // ready for the next step
-  __shared__ EigLds sm;
-  pair_eig(sm, Gpart, nchunk, pi, tol, Qall, skip, offmax,
-           (int)max<int64_t>(0, min<int64_t>(JB, nv - (int64_t)I * JB)),
-           (int)max<int64_t>(0, min<int64_t>(JB, nv - (int64_t)J * JB)));
 }
 
-// Largest normalised off-diagonal over the pair, encoded for atomicMax.
-__device__ __forceinline__ unsigned long long dbits(double x) {
-  return (unsigned long long)__double_as_longlong(x);
-}
-
-// Partner of index x in round r of the 32-position circle schedule.
-__device__ __forceinline__ int rr_partner(int x, int r) {
-  const int n1 = J2 - 1;
-  int pos = x == 0 ? 0 : 1 + ((x - 1 - r) % n1 + n1) % n1;
-  int ppos = n1 - pos;
-  return ppos == 0 ? 0 : 1 + (ppos - 1 + r) % n1;
-}
-
-// Rotation coefficients for index x against partner px: G' = J^T G J with
-// J_ll = J_hh = c, J_lh = s, J_hl = -s (l < h); row x of J^T G is
-// a*G[x] + b*G[px] with a = c and b = -s (x = l) or +s (x = h).
-__device__ __forceinline__ void rot_coef(const double (*G)[J2 + 1], int x, int px, double tol,
-                                         double& a, double& b, bool& rot) {
-  int l = min(x, px), h = max(x, px);
-  double gpq = G[l][h], gpp = G[l][l], gqq = G[h][h];
-  double c = 1.0, sn = 0.0;
-  rot = false;
-  if (gpq != 0.0 && fabs(gpq) > tol * sqrt(fmax(gpp * gqq, 0.0))) {
-    double th = (gqq - gpp) / (2.0 * gpq);
-    double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
-    c = 1.0 / sqrt(1.0 + t * t);
-    sn = t * c;
-    rot = true;
+// G (sm.G[0]) = sum of the pair's nchunk partials (fixed order, so every
+// block that sums them gets the same bits); Q (sm.Q[0]) = I.
+__device__ void eig_load(EigLds& sm, const double* __restrict__ Gp, int nchunk) {
+  for (int e = threadIdx.x; e < J2 * J2; e += 256) {
+    double acc = 0.0;
+    for (int c = 0; c < nchunk; ++c) acc += Gp[(int64_t)c * (J2 * J2) + e];
+    sm.G[0][e / J2][e % J2] = acc;
+    sm.Q[0][e / J2][e % J2] = (e / J2 == e % J2) ? 1.0 : 0.0;
   }
-  a = c;
-  b = x == l ? -sn : sn;
+  __syncthreads();
 }
 
-// Pair eigensolve: sums the pair's Gram partials, records the normalised
-// off-diagonal, diagonalises G = Q diag Q^T in LDS and writes Q (or skip).
+// Pair eigensolve from the partials: records the normalised off-diagonal,
+// diagonalises G = Q diag Q^T in LDS and writes Q (or the skip flag).
 __device__ void pair_eig(EigLds& sm, const double* __restrict__ Gpart, int nchunk, int pi,
                          double tol, double* __restrict__ Qall, int* __restrict__ skip,
-                         unsigned long long* __restrict__ offmax, int nvI, int nvJ) {
-  auto& G = sm.G;
-  auto& Q = sm.Q;
-  auto& red = sm.red;
-  auto& rotated = sm.rotated;
+                         unsigned long long* __restrict__ offmax) {
   const int tid = threadIdx.x;
-  for (int e = tid; e < J2 * J2; e += 256) {
-    double acc = 0.0;
-    for (int c = 0; c < nchunk; ++c) acc += Gpart[((int64_t)pi * nchunk + c) * (J2 * J2) + e];
-    G[0][e / J2][e % J2] = acc;
-    Q[0][e / J2][e % J2] = (e / J2 == e % J2) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  // off-diagonal measure over all pairs p < q of the 32 vectors
-  double off = 0.0;
-  for (int e = tid; e < J2 * J2; e += 256) {
-    int p = e / J2, q = e % J2;
-    if (p < q) {
-      double d = G[0][p][p] * G[0][q][q];
-      if (d > 0.0) off = fmax(off, fabs(G[0][p][q]) / sqrt(d));
-    }
-  }
-  off = wave_max(off);
-  if ((tid & 63) == 0) red[tid >> 6] = off;
-  __syncthreads();
-  off = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  eig_load(sm, Gpart + (int64_t)pi * nchunk * (J2 * J2), nchunk);
+  const double off = eig_off(sm);
   if (tid == 0) atomicMax(offmax, dbits(off));
   if (off <= tol) {
     if (tid == 0) skip[pi] = 1;
     return;
   }
   if (tid == 0) skip[pi] = 0;
-  // Parallel cyclic Jacobi, two barriers per round: 16 threads compute the
-  // round's 16 rotations (FP64) into LDS, then thread t applies the 2x2 block
-  // (row pair t/16, column pair t%16) of G' = J^T G J and Q' = Q J.
-  const double itol = tol;
-  int cur = 0;
-  // sweep flags rotate over 3 slots: slot (s+1)%3 is cleared during sweep s,
-  // so no thread can still be reading it and no thread writes it yet.
-  if (tid == 0) rotated[0] = rotated[1] = rotated[2] = 0;
-  __syncthreads();
-  const int rp = tid >> 4, cp = tid & 15;
-  // One inner sweep per outer step: the outer block iteration needs the
-  // same number of sweeps with one inner sweep as with full inner
-  // convergence (measured), at a fraction of the cost.
-  for (int sweep = 0; sweep < 1; ++sweep) {
-    const int slot = sweep % 3;
-    if (tid == 0) rotated[(sweep + 1) % 3] = 0;
-    for (int r = 0; r < J2 - 1; ++r) {
-      const int nxt = cur ^ 1;
-      if (tid < J2 / 2) {
-        int x0 = tid == 0 ? 0 : 1 + (tid - 1 + r) % (J2 - 1);
-        int x1 = 1 + (J2 - 2 - tid + r) % (J2 - 1);
-        double a0, b0;
-        bool rx;
-        rot_coef(G[cur], x0, x1, itol, a0, b0, rx);  // a0 = c, b0 = -+s for x0
-        sm.cs[tid][0] = a0;
-        sm.cs[tid][1] = b0;
-        if (rx) rotated[slot] = 1;
-      }
-      __syncthreads();
-      int x0 = rp == 0 ? 0 : 1 + (rp - 1 + r) % (J2 - 1);
-      int x1 = 1 + (J2 - 2 - rp + r) % (J2 - 1);
-      int y0 = cp == 0 ? 0 : 1 + (cp - 1 + r) % (J2 - 1);
-      int y1 = 1 + (J2 - 2 - cp + r) % (J2 - 1);
-      // cs holds (c, b) for the first index of each pair; the partner's b is -b
-      const double cx = sm.cs[rp][0], sx = sm.cs[rp][1];
-      const double cy = sm.cs[cp][0], sy = sm.cs[cp][1];
-      const double ax0 = cx, ax1 = cx;
-      const double bX0 = sx, bX1 = -sx, bY0 = sy, bY1 = -sy;
-      const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
-      const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
-      const double r00 = ax0 * g00 + bX0 * g10, r01 = ax0 * g01 + bX0 * g11;
-      const double r10 = ax1 * g10 + bX1 * g00, r11 = ax1 * g11 + bX1 * g01;
-      G[nxt][x0][y0] = cy * r00 + bY0 * r01;
-      G[nxt][x0][y1] = cy * r01 + bY1 * r00;
-      G[nxt][x1][y0] = cy * r10 + bY0 * r11;
-      G[nxt][x1][y1] = cy * r11 + bY1 * r10;
-      const double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
-      const double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
-      Q[nxt][x0][y0] = cy * q00 + bY0 * q01;
-      Q[nxt][x0][y1] = cy * q01 + bY1 * q00;
-      Q[nxt][x1][y0] = cy * q10 + bY0 * q11;
-      Q[nxt][x1][y1] = cy * q11 + bY1 * q10;
-      cur = nxt;
-      __syncthreads();
-    }
-    const int again = rotated[slot];
-    if (!again) break;
-  }
+  const int cur = eig_rotate(sm, tol);
   double* out = Qall + (int64_t)pi * (J2 * J2);
-  for (int e = tid; e < J2 * J2; e += 256) out[e] = Q[cur][e / J2][e % J2];
+  for (int e = tid; e < J2 * J2; e += 256) out[e] = sm.Q[cur][e / J2][e % J2];
+}
+
+__global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, int64_t L,
+                                                int64_t nv, int s, int nblk, int nchunk,
+                                                double* __restrict__ Gpart, int* counter,
+                                                double tol, double* __restrict__ Qall,
+                                                int* __restrict__ skip,
+                                                unsigned long long* __restrict__ offmax) {
+  __shared__ double Ps[2][64][J2 + 1];
+  const int pi = blockIdx.x, ch = blockIdx.y;
+  int I, J;
+  pair_blocks(pi, s, nblk, I, J);
+  const int64_t r0 = (int64_t)ch * JCH;
+  const int64_t r1 = min(L, r0 + JCH);
+  gram_partial(Ps, Wv, L, nv, I, J, r0, r1, Gpart + ((int64_t)pi * nchunk + ch) * (J2 * J2));
+  if (counter == nullptr) return;
+  // Last-arriving chunk block of the pair runs the pair eigensolve: release
+  // the partial device-wide, count arrivals, acquire before reading partials.
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = (atomicAdd(counter + pi, 1) == nchunk - 1) ? 1 : 0;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) counter[pi] = 0;  // ready for the next step
+  __shared__ EigLds sm;
+  pair_eig(sm, Gpart, nchunk, pi, tol, Qall, skip, offmax);
 }
 
 __global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart, int nchunk,
-                                               int64_t nv, int s, int nblk, double tol,
-                                               double* __restrict__ Qall, int* __restrict__ skip,
+                                               double tol, double* __restrict__ Qall,
+                                               int* __restrict__ skip,
                                                unsigned long long* __restrict__ offmax) {
   __shared__ EigLds sm;
-  int I, J;
-  pair_blocks(blockIdx.x, s, nblk, I, J);
-  pair_eig(sm, Gpart, nchunk, blockIdx.x, tol, Qall, skip, offmax,
-           (int)max<int64_t>(0, min<int64_t>(JB, nv - (int64_t)I * JB)),
-           (int)max<int64_t>(0, min<int64_t>(JB, nv - (int64_t)J * JB)));
+  pair_eig(sm, Gpart, nchunk, blockIdx.x, tol, Qall, skip, offmax);
 }
 
 // rows [ch*JCH, ...) of the stacked [Wv (L rows); Z (nv rows)] for pair pi
@@ -280,36 +155,19 @@ __global__ void __launch_bounds__(256) jac_apply(double* __restrict__ Wv, int64_
                                                  double* __restrict__ Z, int64_t nv, int s,
                                                  int nblk, int nchW, const double* __restrict__ Qall,
                                                  const int* __restrict__ skip) {
-  __shared__ double Q[J2][J2];
+  __shared__ double Q[J2][J2 + 1];
   const int pi = blockIdx.x, ch = blockIdx.y;
   if (skip[pi]) return;
   int I, J;
   pair_blocks(pi, s, nblk, I, J);
   for (int e = threadIdx.x; e < J2 * J2; e += 256) Q[e / J2][e % J2] = Qall[(int64_t)pi * J2 * J2 + e];
   __syncthreads();
-  double* base;
-  int64_t ld, r;
   if (ch < nchW) {
-    base = Wv; ld = L; r = (int64_t)ch * JCH + threadIdx.x;
-    if (r >= L) return;
+    const int64_t r = (int64_t)ch * JCH;
+    apply_mfma(Wv, L, nv, r, min(L, r + JCH), I, J, Q);
   } else {
-    base = Z; ld = nv; r = (int64_t)(ch - nchW) * JCH + threadIdx.x;
-    if (r >= nv) return;
-  }
-  double x[J2];
-#pragma unroll
-  for (int c = 0; c < J2; ++c) {
-    int64_t v = pair_vec(c, I, J);
-    x[c] = v < nv ? base[v * ld + r] : 0.0;
-  }
-#pragma unroll
-  for (int c = 0; c < J2; ++c) {
-    int64_t v = pair_vec(c, I, J);
-    if (v >= nv) continue;
-    double acc = 0.0;
-#pragma unroll
-    for (int i = 0; i < J2; ++i) acc = fma(x[i], Q[i][c], acc);
-    base[v * ld + r] = acc;
+    const int64_t r = (int64_t)(ch - nchW) * JCH;
+    apply_mfma(Z, nv, nv, r, min(nv, r + JCH), I, J, Q);
   }
 }
 
@@ -350,11 +208,15 @@ __global__ void __launch_bounds__(1024) rank_sort(const double* __restrict__ sig
 // U (m x k row-major) and Vt (k x n row-major).  Column j of U is vector
 // perm[j] of Uvec (ld ldu), row j of Vt is vector perm[j] of Vvec (ld ldv);
 // a "scaled" source holds sigma_j * (unit vector) and is divided by S[j].
+// upm / vpm (nullable): component i of a U source vector is row upm[i] of U,
+// component c of a V source vector is column vpm[c] of Vt (the pivot order
+// of the preconditioned path).
 template <typename T>
 __global__ void svd_out_kernel(const double* __restrict__ Uvec, int64_t ldu, int u_scaled,
                                const double* __restrict__ Vvec, int64_t ldv, int v_scaled,
                                int64_t k, const int* __restrict__ perm,
-                               const double* __restrict__ S, T* __restrict__ U,
+                               const double* __restrict__ S, const int* __restrict__ upm,
+                               const int* __restrict__ vpm, T* __restrict__ U,
                                T* __restrict__ Vt, int64_t m, int64_t n) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= m * k + k * n) return;
@@ -362,14 +224,25 @@ __global__ void svd_out_kernel(const double* __restrict__ Uvec, int64_t ldu, int
     int64_t i = idx / k, j = idx - i * k;
     double v = Uvec[(int64_t)perm[j] * ldu + i];
     if (u_scaled) v = S[j] > 0.0 ? v / S[j] : 0.0;
-    U[idx] = (T)v;
+    const int64_t row = upm ? (int64_t)upm[i] : i;
+    U[row * k + j] = (T)v;
   } else {
     int64_t e = idx - m * k;
     int64_t j = e / n, c = e - j * n;
     double v = Vvec[(int64_t)perm[j] * ldv + c];
     if (v_scaled) v = S[j] > 0.0 ? v / S[j] : 0.0;
-    Vt[e] = (T)v;
+    const int64_t col = vpm ? (int64_t)vpm[c] : c;
+    Vt[j * n + col] = (T)v;
   }
+}
+
+// Xp[c*L + i] = X[p[c]*L + i]  (k vectors of length L)
+__global__ void permute_vectors(const double* __restrict__ X, int64_t L, int64_t k,
+                                const int* __restrict__ p, double* __restrict__ Xp) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= L * k) return;
+  const int64_t c = idx / L, i = idx - c * L;
+  Xp[idx] = X[(int64_t)p[c] * L + i];
 }
 
 // Rows of the upper-triangular R (in the column-major QR workspace, ld = L)
@@ -405,7 +278,7 @@ __global__ void init_identity(double* __restrict__ Z, int64_t k) {
 
 struct SvdWs {
   double *Wv, *Z, *Gpart, *Q, *sig, *S, *R, *QZ;
-  int *skip, *perm, *counter;
+  int *skip, *perm, *counter, *vp;
   unsigned long long* off;
   QrWs qr;
   int precond;
@@ -435,6 +308,7 @@ static size_t svd_layout(int64_t m, int64_t n, char* base, size_t cap, SvdWs* ou
   if (w.precond) {
     w.R = a.take<double>((size_t)(k * k));
     w.QZ = a.take<double>((size_t)(L * k));
+    w.vp = a.take<int>((size_t)k);
     size_t qbytes = qr_layout(L, k, nullptr, 0, nullptr) + 256;
     char* qbase = a.take<char>(qbytes);
     if (base) qr_layout(L, k, qbase, qbytes, &w.qr);
@@ -456,13 +330,13 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
   if (Z)
     hipLaunchKernelGGL(init_identity, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, Z, k);
   TN_CHECK_LAUNCH();
-  TN_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * npairs, st));
   int sweep = 0;
   *converged = false;
-  unsigned long long host_off = 0;
   // algorithmic work per step: Gram 2 L (2b)^2 per pair, apply 2 (L + k) (2b)^2 per pair
   const double gram_fl = 2.0 * (double)L * J2 * J2 * npairs;
   const double apply_fl = 2.0 * (double)(L + kz) * J2 * J2 * npairs;
+  TN_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * npairs, st));
+  unsigned long long host_off = 0;
   auto enqueue_sweep = [&]() -> int {
     hipLaunchKernelGGL(reset_u64, dim3(1), dim3(1), 0, st, w.off);
     for (int s = 0; s < nblk - 1; ++s) {
@@ -578,21 +452,31 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
   int u_scaled, v_scaled;
   const double* orth_src;  // vectors whose norms are the singular values
   int64_t orth_len;
+  const int* upm = nullptr;
+  const int* vpm = nullptr;
   if (w.precond) {
-    // X = Q R; one-sided Jacobi on the rows of R (R^T Z = W''), then
-    // X = (Q Z) Sigma (W''/Sigma)^T.
-    TN_TRY(qr_factor(w.Wv, L, k, w.qr, st));
-    hipLaunchKernelGGL(r_rows_kernel, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, w.Wv,
+    // Static pivot: X' = X P with the vectors by descending norm (R R^T is
+    // then closer to diagonal and the Jacobi needs fewer sweeps).
+    // X' = Q R; one-sided Jacobi on the rows of R (R^T Z = W''), then
+    // X' = (Q Z) Sigma (W''/Sigma)^T and X = X' P^T.
+    hipLaunchKernelGGL(vec_norms, dim3((unsigned)k), dim3(256), 0, st, w.Wv, L, w.sig);
+    hipLaunchKernelGGL(rank_sort, dim3(1), dim3(1024), 0, st, w.sig, k, w.vp, w.S);
+    hipLaunchKernelGGL(permute_vectors, dim3((unsigned)ceil_div(L * k, 256)), dim3(256), 0, st,
+                       w.Wv, L, k, w.vp, w.QZ);
+    TN_CHECK_LAUNCH();
+    TN_TRY(qr_factor(w.QZ, L, k, w.qr, st));
+    hipLaunchKernelGGL(r_rows_kernel, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, w.QZ,
                        L, k, w.R);
     TN_CHECK_LAUNCH();
     if (tol <= 0.0) tol = std::max(1e-15, (double)k * 2.220446049250313e-16);
     TN_TRY(jacobi_run(w.R, k, w.Z, k, w, tol, max_sweeps, sweeps_out, &converged, st));
-    TN_TRY(qr_form_q(w.Wv, L, k, w.qr, w.QZ, st, w.Z));
+    TN_TRY(qr_form_q(w.QZ, L, k, w.qr, w.Wv, st, w.Z));
     orth_src = w.R;
     orth_len = k;
-    // left vectors of X: Q Z (orthonormal); right vectors of X: rows of R rotated
-    if (tall) { Uvec = w.QZ; ldu = L; u_scaled = 0; Vvec = w.R; ldv = k; v_scaled = 1; }
-    else { Uvec = w.R; ldu = k; u_scaled = 1; Vvec = w.QZ; ldv = L; v_scaled = 0; }
+    // left vectors of X: Q Z (orthonormal); right vectors of X: rows of R
+    // rotated, whose components are in pivot order
+    if (tall) { Uvec = w.Wv; ldu = L; u_scaled = 0; Vvec = w.R; ldv = k; v_scaled = 1; vpm = w.vp; }
+    else { Uvec = w.R; ldu = k; u_scaled = 1; Vvec = w.Wv; ldv = L; v_scaled = 0; upm = w.vp; }
   } else {
     // LAPACK dgesvj convergence threshold: (vector length) * eps
     if (tol <= 0.0) tol = std::max(1e-15, (double)L * 2.220446049250313e-16);
@@ -608,10 +492,12 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
   unsigned ob = (unsigned)ceil_div(m * k + k * n, 256);
   if (dtype == TN_F64)
     hipLaunchKernelGGL(svd_out_kernel<double>, dim3(ob), dim3(256), 0, st, Uvec, ldu, u_scaled,
-                       Vvec, ldv, v_scaled, k, w.perm, S, (double*)U, (double*)Vt, m, n);
+                       Vvec, ldv, v_scaled, k, w.perm, S, upm, vpm, (double*)U, (double*)Vt, m,
+                       n);
   else
     hipLaunchKernelGGL(svd_out_kernel<float>, dim3(ob), dim3(256), 0, st, Uvec, ldu, u_scaled,
-                       Vvec, ldv, v_scaled, k, w.perm, S, (float*)U, (float*)Vt, m, n);
+                       Vvec, ldv, v_scaled, k, w.perm, S, upm, vpm, (float*)U, (float*)Vt, m,
+                       n);
   TN_CHECK_LAUNCH();
   if (!converged) {
     set_error(TN_ERR_NOCONV, "tn_svd: Jacobi did not converge in %d sweeps", max_sweeps);
@@ -632,9 +518,9 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
 //   1. P orders K by descending diagonal (a static pivot); shift
 //      delta = 8 sqrt(n) eps_in tr(K), eps_in the input's unit roundoff;
 //   2. P K P^T + delta I = L L^T, blocked right-looking Cholesky (64-column
-//      blocks: the diagonal block in LDS, the panel by a GEMM with the
-//      inverted diagonal block, the trailing update by an MFMA GEMM; a
-//      column whose pivot falls below delta/2 is dropped);
+//      blocks: the diagonal block in one wavefront's registers, the panel
+//      by row-parallel forward substitution, the trailing update by an MFMA
+//      GEMM; a column whose pivot falls below delta/2 is dropped);
 //   3. one-sided block Jacobi on the columns of L (no rotation accumulation):
 //      L Z = U' Sigma, hence P K P^T + delta I = U' Sigma^2 U'^T, so
 //      U = P^T U' and lam = sigma^2 - delta.
@@ -679,56 +565,81 @@ __global__ void psd_permute(const double* __restrict__ A, int64_t n, const int* 
   B[idx] = v;
 }
 
-// Factor the w x w diagonal block at (j0, j0) of B (column-major, ld n):
-// writes L11 (lower, zeros above) into Lw at the same position and
-// T = L11^{-T} (row-major CB x CB) into Tinv.  One wavefront: thread t owns
-// row t of the block.
+// Factor the w x w diagonal block at (j0, j0) of B (column-major, ld n) and
+// write L11 (lower, zeros above) into Lw at the same position.  One
+// wavefront; lane t keeps row t of the block in registers; column c's
+// entries are exchanged through LDS (two wave-local barriers per column).
 __global__ void __launch_bounds__(64) chol_diag(const double* __restrict__ B, int64_t n,
                                                 int64_t j0, int w,
                                                 const double* __restrict__ delta,
-                                                double* __restrict__ Lw,
-                                                double* __restrict__ Tinv) {
-  __shared__ double a[CB][CB + 1];
-  __shared__ double x[CB][CB + 1];
+                                                double* __restrict__ Lw) {
+  __shared__ double col[CB];
   const int t = threadIdx.x;
   const double dl = *delta;
-  for (int c = 0; c < w; ++c) a[t][c] = t < w ? B[(j0 + c) * n + j0 + t] : 0.0;
-  __syncthreads();
-  for (int c = 0; c < w; ++c) {
+  double r[CB];
+#pragma unroll
+  for (int c = 0; c < CB; ++c) r[c] = (t < w && c < w) ? B[(j0 + c) * n + j0 + t] : 0.0;
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    if (c >= w) break;
+    if (t == c) col[0] = r[c];
+    __syncthreads();
     // With the shift every exact pivot is >= delta; one below delta/2 means
     // the input is indefinite beyond its rounding allowance: the column is
     // dropped (zero) rather than divided by a tiny pivot.
-    const double d = a[c][c];
+    const double d = col[0];
     const double s = d >= 0.5 * dl && d > 0.0 ? sqrt(d) : 0.0;
     const double inv = s > 0.0 ? 1.0 / s : 0.0;
-    __syncthreads();  // a[c][c] read by every thread before it is replaced
-    if (t == c) a[c][c] = s;
-    if (t > c && t < w) a[t][c] *= inv;
+    if (t == c) r[c] = s;
+    if (t > c) r[c] *= inv;
+    __syncthreads();  // col[0] read by every lane before it is replaced
+    col[t] = t > c ? r[c] : 0.0;
     __syncthreads();
-    if (t > c && t < w) {
-      const double l = a[t][c];
-      for (int c2 = c + 1; c2 <= t; ++c2) a[t][c2] -= l * a[c2][c];
+    if (t > c) {
+#pragma unroll
+      for (int c2 = c + 1; c2 < CB; ++c2) r[c2] -= r[c] * col[c2];
     }
     __syncthreads();
   }
-  // X = L11^{-1}, column t by forward substitution
   if (t < w) {
-    for (int i = 0; i < w; ++i) {
-      double v = 0.0;
-      if (i >= t) {
-        double acc = i == t ? 1.0 : 0.0;
-        for (int k = t; k < i; ++k) acc -= a[i][k] * x[k][t];
-        const double dd = a[i][i];
-        v = dd > 0.0 ? acc / dd : 0.0;
-      }
-      x[i][t] = v;
-    }
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+      if (c < w) Lw[(j0 + c) * n + j0 + t] = t >= c ? r[c] : 0.0;
+  }
+}
+
+// Panel below the diagonal block: L21 = A21 L11^{-T} (forward substitution
+// per row, one thread per row, the row in registers, L11 in LDS).  Rows
+// [j0 + w, n) of columns [j0, j0 + w); A21 from B, L11 and L21 in Lw.
+__global__ void __launch_bounds__(256) chol_panel(const double* __restrict__ B, int64_t n,
+                                                  int64_t j0, int w, double* __restrict__ Lw) {
+  __shared__ double Ls[CB][CB + 1];
+  __shared__ double rd[CB];
+  for (int e = threadIdx.x; e < CB * CB; e += 256) {
+    const int c = e / CB, rr = e % CB;  // column c, row rr of L11
+    Ls[rr][c] = (c < w && rr < w) ? Lw[(j0 + c) * n + j0 + rr] : 0.0;
   }
   __syncthreads();
-  if (t < w) {
-    for (int c = 0; c < w; ++c) Lw[(j0 + c) * n + j0 + t] = t >= c ? a[t][c] : 0.0;
-    for (int c = 0; c < w; ++c) Tinv[t * CB + c] = x[c][t];  // T[t][c] = X[c][t]
+  if (threadIdx.x < CB) {
+    const double dd = Ls[threadIdx.x][threadIdx.x];
+    rd[threadIdx.x] = dd > 0.0 ? 1.0 / dd : 0.0;
   }
+  __syncthreads();
+  const int64_t i = j0 + w + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double x[CB];
+#pragma unroll
+  for (int c = 0; c < CB; ++c) x[c] = c < w ? B[(j0 + c) * n + i] : 0.0;
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    double acc = x[c];
+#pragma unroll
+    for (int j = 0; j < c; ++j) acc -= x[j] * Ls[c][j];
+    x[c] = acc * rd[c];
+  }
+#pragma unroll
+  for (int c = 0; c < CB; ++c)
+    if (c < w) Lw[(j0 + c) * n + i] = x[c];
 }
 
 // U[p[i]][j] = Wv[perm[j]][i] / S[j];  lam[j] = S[j]^2 - delta
@@ -746,7 +657,7 @@ __global__ void eig_out_kernel(const double* __restrict__ Wv, int64_t n, const i
 
 struct EigWs {
   SvdWs jw;  // Jacobi buffers (Wv holds L)
-  double *B, *Tinv, *diag, *Sd, *delta;
+  double *B, *diag, *Sd, *delta;
   int* p;
 };
 
@@ -758,7 +669,6 @@ static size_t eig_layout(int64_t n, char* base, size_t cap, EigWs* out) {
   Arena a{base, cap, 0};
   e.jw.Wv = a.take<double>((size_t)(n * n));
   e.B = a.take<double>((size_t)(n * n));
-  e.Tinv = a.take<double>((size_t)CB * CB);
   e.diag = a.take<double>((size_t)n);
   e.Sd = a.take<double>((size_t)n);
   e.delta = a.take<double>(4);
@@ -820,14 +730,14 @@ extern "C" int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes
   // 2. blocked Cholesky: B (working copy) -> L in Wv
   for (int64_t j0 = 0; j0 < n; j0 += CB) {
     const int w = (int)std::min<int64_t>(CB, n - j0);
-    hipLaunchKernelGGL(chol_diag, dim3(1), dim3(64), 0, st, e.B, n, j0, w, e.delta, Wv, e.Tinv);
+    hipLaunchKernelGGL(chol_diag, dim3(1), dim3(64), 0, st, e.B, n, j0, w, e.delta, Wv);
     TN_CHECK_LAUNCH();
     const int64_t m2 = n - j0 - w;
     if (m2 <= 0) break;
-    // L21 = A21 T   (m2 x w)
-    TN_TRY(contract_launch(e.B + j0 * n + j0 + w, mk({m2}, {1}), mk({w}, {n}), e.Tinv,
-                           mk({w}, {CB}), mk({w}, {1}), Wv + j0 * n + j0 + w, mk({m2}, {1}),
-                           mk({w}, {n}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, nullptr, 0, st));
+    // L21 = A21 L11^{-T}   (m2 x w)
+    hipLaunchKernelGGL(chol_panel, dim3((unsigned)ceil_div(m2, 256)), dim3(256), 0, st, e.B, n, j0,
+                       w, Wv);
+    TN_CHECK_LAUNCH();
     // A22 -= L21 L21^T
     const double* L21 = Wv + j0 * n + j0 + w;
     TN_TRY(contract_launch(L21, mk({m2}, {1}), mk({w}, {n}), L21, mk({w}, {n}), mk({m2}, {1}),
