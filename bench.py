@@ -234,6 +234,7 @@ def main():
         "speedup_vs_plain_fp64": (dt_plain / dt_fast) if dt_plain else None,
         "state_err_vs_plain_fp64": err,
         "kernel_ms_profiled_sweep": kernel_ms,
+        "eig_jacobi_sweeps_mean": (sum(st.sweeps.values()) / max(len(st.sweeps), 1)),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

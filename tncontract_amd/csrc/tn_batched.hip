@@ -211,6 +211,7 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
                                                   const double* __restrict__ Qall,
                                                   const int* __restrict__ skip) {
   __shared__ double Q[BJ2][BJ2 + 1];
+  __shared__ ApplyTile Tw;
   const int item = blockIdx.x;
   const int j = a_job[item];
   if (done[j]) return;
@@ -220,14 +221,13 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
   if (skip[eidx]) return;
   const int s = g % (J.nblk - 1);
   const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
-  for (int e = threadIdx.x; e < BJ2 * BJ2; e += 256) Q[e / BJ2][e % BJ2] = Qall[(int64_t)eidx * BJ2 * BJ2 + e];
-  __syncthreads();
+  const double* Qg = Qall + (int64_t)eidx * BJ2 * BJ2;
   if (ch < J.nch) {
     const int64_t r = (int64_t)ch * BCH;
-    apply_mfma(Wv + J.wv, J.L, J.k, r, min((int64_t)J.L, r + BCH), I, K2, Q);
+    apply_mfma(Wv + J.wv, J.L, J.k, r, min((int64_t)J.L, r + BCH), I, K2, Qg, Q, Tw);
   } else {
     const int64_t r = (int64_t)(ch - J.nch) * BCH;
-    apply_mfma(Zb + J.z, J.k, J.k, r, min((int64_t)J.k, r + BCH), I, K2, Q);
+    apply_mfma(Zb + J.z, J.k, J.k, r, min((int64_t)J.k, r + BCH), I, K2, Qg, Q, Tw);
   }
 }
 

@@ -58,16 +58,9 @@ __device__ inline double eig_off(EigLds& sm) {
   return off;
 }
 
-// 1/x and 1/sqrt(x) for FP64 from the hardware estimates (v_rcp_f64,
-// v_rsq_f64) refined by two Newton steps to full double precision: far
-// shorter dependent chains than IEEE division and square root.
-__device__ __forceinline__ double rcp_nr(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
-}
+// 1/sqrt(x) for FP64 from the hardware estimate (v_rsq_f64) refined by two
+// Newton steps to full double precision: a far shorter dependent chain than
+// an IEEE square root and division.
 __device__ __forceinline__ double rsq_nr(double x) {
   double y = __builtin_amdgcn_rsq(x);
   const double hx = 0.5 * x;
@@ -81,27 +74,32 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // J_ll = J_hh = c, J_lh = s, J_hl = -s (l < h); row x of J^T G is
 // a*G[x] + b*G[px] with a = c and b = -s (x = l) or +s (x = h).
 // t = tan(angle) = sgn(D P) |P| / (|D| + sqrt(D^2 + P^2)), D = gqq - gpp,
-// P = 2 gpq, evaluated on D, P scaled by 1/max(|D|, |P|); c = 1/sqrt(1 + t^2),
-// s = t c.  A pair is rotated when gpq^2 > tol^2 gpp gqq (then |gpq| is far
-// above the underflow range, so the scaling is finite).
+// P = 2 gpq.  The angle only has to shrink the off-diagonal, so t is
+// evaluated in FP32 on D, P scaled by a power of two (exact) into [0.5, 1):
+// a 2^-24 error in t leaves a 2^-24 fraction of G_lh, which the next sweeps
+// remove like any other (same sweep counts as an FP64 angle, measured).
+// Orthogonality comes from c = 1/sqrt(1 + t^2) in FP64 (v_rsq_f64 + two
+// Newton steps) and s = t c, so c^2 + s^2 = 1 to FP64 rounding.  A pair is
+// rotated when gpq^2 > tol^2 gpp gqq; the coefficients are computed without
+// a branch (a pair that is not rotated selects c = 1, s = 0, discarding
+// whatever the formula gave), so the two rotations a thread needs per round
+// overlap.
 __device__ __forceinline__ void rot_coef(const double (*G)[J2 + 1], int x, int px, double tol,
                                          double& a, double& b) {
   const int l = min(x, px), h = max(x, px);
   const double gpq = G[l][h], gpp = G[l][l], gqq = G[h][h];
-  double c = 1.0, sn = 0.0;
-  if (gpq * gpq > tol * tol * fmax(gpp * gqq, 0.0)) {
-    const double D = gqq - gpp, P = 2.0 * gpq;
-    const double aD = fabs(D), aP = fabs(P);
-    const double rm = rcp_nr(fmax(aD, aP));
-    const double d = aD * rm, p = aP * rm;
-    const double q = fma(d, d, p * p);
-    const double hyp = q * rsq_nr(q);
-    const double t = ((D * P >= 0.0) ? p : -p) * rcp_nr(d + hyp);
-    c = rsq_nr(fma(t, t, 1.0));
-    sn = t * c;
-  }
-  a = c;
-  b = x == l ? -sn : sn;
+  const bool rot = gpq * gpq > tol * tol * fmax(gpp * gqq, 0.0);
+  const double D = gqq - gpp, P = 2.0 * gpq;
+  const int e = __builtin_amdgcn_frexp_exp(fmax(fabs(D), fabs(P)));
+  const float d = (float)__builtin_amdgcn_ldexp(fabs(D), -e);
+  const float p = (float)__builtin_amdgcn_ldexp(fabs(P), -e);
+  const float hyp = __builtin_amdgcn_sqrtf(fmaf(d, d, p * p));
+  const float t32 = p * __builtin_amdgcn_rcpf(d + hyp);
+  const double t = (D * P >= 0.0) ? (double)t32 : -(double)t32;
+  const double c = rsq_nr(fma(t, t, 1.0));
+  const double sn = t * c;
+  a = rot ? c : 1.0;
+  b = rot ? (x == l ? -sn : sn) : 0.0;
 }
 
 // One cyclic sweep (31 rounds of 16 disjoint rotations, circle order) of the
@@ -152,27 +150,39 @@ __device__ inline int eig_rotate(EigLds& sm, double tol) {
   return cur;
 }
 
+// Per-wave LDS staging tiles of apply_mfma (64 rows x 32 vectors per wave).
+typedef double ApplyTile[4][64][J2 + 1];
+
 // Rows [rs, re) of the pair's 32 vectors (vector v at base + v * ld, nv
-// vectors) <- rows . Q, Q 32 x 32 in LDS (row stride J2 + 1).  Each wave
-// takes 64 rows per pass: it loads its 64 x 32 tile into registers (the A
-// operands of v_mfma_f64_16x16x4), multiplies by Q (B operands from LDS) and
-// stores the tile back; every store depends on all loads of its rows, so the
-// update is in place.  256 threads.
+// vectors) <- rows . Q, Q 32 x 32 (row-major Qg in global memory, staged in
+// the LDS buffer Q, row stride J2 + 1).  Each wave
+// takes 64 rows per pass: it loads the 64 x 32 tile column by column (64
+// consecutive rows per load) into its LDS tile, multiplies it by Q with
+// v_mfma_f64_16x16x4 (A operands from the tile, B from Q), writes the
+// product back into the tile and stores it column by column.  The pass
+// count is the same for every wave (block-wide barriers), 256 threads.
 __device__ inline void apply_mfma(double* __restrict__ base, int64_t ld, int64_t nv, int64_t rs,
-                           int64_t re, int I, int J, const double (*Q)[J2 + 1]) {
+                                  int64_t re, int I, int J, const double* __restrict__ Qg,
+                                  double (*Q)[J2 + 1], ApplyTile& Tw) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
-  for (int64_t r0 = rs + 64 * wave; r0 < re; r0 += 256) {
+  auto& T = Tw[wave];
+  for (int64_t rb = rs; rb < re; rb += 256) {
+    const int64_t r0 = rb + 64 * wave;
+    const int64_t r = r0 + lane;
+#pragma unroll
+    for (int c = 0; c < J2; ++c) {
+      const int64_t v = pair_vec(c, I, J);
+      T[lane][c] = (v < nv && r < re) ? base[v * ld + r] : 0.0;
+    }
+    if (rb == rs)  // Q (global, 32 x 32) -> LDS, its loads in flight with the tile's
+      for (int e = threadIdx.x; e < J2 * J2; e += 256) Q[e / J2][e % J2] = Qg[e];
+    __syncthreads();
     double a[4][8];
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-      const int64_t v = pair_vec(4 * ks + lk, I, J);
+    for (int ks = 0; ks < 8; ++ks)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int64_t r = r0 + 16 * t + lr;
-        a[t][ks] = (v < nv && r < re) ? base[v * ld + r] : 0.0;
-      }
-    }
+      for (int t = 0; t < 4; ++t) a[t][ks] = T[16 * t + lr][4 * ks + lk];
     f64x4 acc[4][2];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = f64x4{0, 0, 0, 0};
@@ -186,18 +196,20 @@ __device__ inline void apply_mfma(double* __restrict__ base, int64_t ld, int64_t
           acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][ks], b, acc[t][ct], 0, 0, 0);
       }
     }
+    __syncthreads();
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct) {
-      const int64_t v = pair_vec(16 * ct + lr, I, J);
-      if (v >= nv) continue;
+    for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
       for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int64_t r = r0 + 16 * t + lk + 4 * e;
-          if (r < re) base[v * ld + r] = acc[t][ct][e];
-        }
+        for (int e = 0; e < 4; ++e) T[16 * t + lk + 4 * e][16 * ct + lr] = acc[t][ct][e];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < J2; ++c) {
+      const int64_t v = pair_vec(c, I, J);
+      if (v < nv && r < re) base[v * ld + r] = T[lane][c];
     }
+    __syncthreads();
   }
 }
 

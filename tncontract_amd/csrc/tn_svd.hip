@@ -10,11 +10,13 @@
 //   1. jac_gram:  partial Gram G = P^T P of P = [W_I W_J] over a row chunk,
 //                 v_mfma_f64_16x16x4 from an LDS tile (grid: pairs x chunks);
 //   2. jac_eig:   sums the partials, records the pair's largest normalised
-//                 off-diagonal |G_pq|/sqrt(G_pp G_qq), and diagonalises the
-//                 32x32 G with a parallel cyclic two-sided Jacobi in LDS
-//                 (G = Q diag Q^T);
-//   3. jac_apply: [W_I W_J] <- [W_I W_J] Q and [Z_I Z_J] <- [Z_I Z_J] Q,
-//                 row-parallel (grid: pairs x row chunks).
+//                 off-diagonal |G_pq|/sqrt(G_pp G_qq), and runs one cyclic
+//                 sweep of two-sided Jacobi on the 32x32 G in LDS,
+//                 accumulating Q (grid: pairs);
+//   3. jac_apply: [W_I W_J] <- [W_I W_J] Q and [Z_I Z_J] <- [Z_I Z_J] Q by
+//                 MFMA (grid: pairs x row chunks).
+// The three kernels of every step of a sweep are captured once as a HIP
+// graph and replayed.
 // A sweep is nblk-1 steps; sweeps repeat until the largest normalised
 // off-diagonal of a whole sweep is <= tol.  Singular values are the vector
 // norms; they are sorted descending on the device.
@@ -116,10 +118,7 @@ __device__ void pair_eig(EigLds& sm, const double* __restrict__ Gpart, int nchun
 
 __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, int64_t L,
                                                 int64_t nv, int s, int nblk, int nchunk,
-                                                double* __restrict__ Gpart, int* counter,
-                                                double tol, double* __restrict__ Qall,
-                                                int* __restrict__ skip,
-                                                unsigned long long* __restrict__ offmax) {
+                                                double* __restrict__ Gpart) {
   __shared__ double Ps[2][64][J2 + 1];
   const int pi = blockIdx.x, ch = blockIdx.y;
   int I, J;
@@ -127,19 +126,6 @@ __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, i
   const int64_t r0 = (int64_t)ch * JCH;
   const int64_t r1 = min(L, r0 + JCH);
   gram_partial(Ps, Wv, L, nv, I, J, r0, r1, Gpart + ((int64_t)pi * nchunk + ch) * (J2 * J2));
-  if (counter == nullptr) return;
-  // Last-arriving chunk block of the pair runs the pair eigensolve: release
-  // the partial device-wide, count arrivals, acquire before reading partials.
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(counter + pi, 1) == nchunk - 1) ? 1 : 0;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (threadIdx.x == 0) counter[pi] = 0;  // ready for the next step
-  __shared__ EigLds sm;
-  pair_eig(sm, Gpart, nchunk, pi, tol, Qall, skip, offmax);
 }
 
 __global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart, int nchunk,
@@ -156,18 +142,18 @@ __global__ void __launch_bounds__(256) jac_apply(double* __restrict__ Wv, int64_
                                                  int nblk, int nchW, const double* __restrict__ Qall,
                                                  const int* __restrict__ skip) {
   __shared__ double Q[J2][J2 + 1];
+  __shared__ ApplyTile Tw;
   const int pi = blockIdx.x, ch = blockIdx.y;
   if (skip[pi]) return;
   int I, J;
   pair_blocks(pi, s, nblk, I, J);
-  for (int e = threadIdx.x; e < J2 * J2; e += 256) Q[e / J2][e % J2] = Qall[(int64_t)pi * J2 * J2 + e];
-  __syncthreads();
+  const double* Qg = Qall + (int64_t)pi * J2 * J2;
   if (ch < nchW) {
     const int64_t r = (int64_t)ch * JCH;
-    apply_mfma(Wv, L, nv, r, min(L, r + JCH), I, J, Q);
+    apply_mfma(Wv, L, nv, r, min(L, r + JCH), I, J, Qg, Q, Tw);
   } else {
     const int64_t r = (int64_t)(ch - nchW) * JCH;
-    apply_mfma(Z, nv, nv, r, min(nv, r + JCH), I, J, Q);
+    apply_mfma(Z, nv, nv, r, min(nv, r + JCH), I, J, Qg, Q, Tw);
   }
 }
 
@@ -278,7 +264,7 @@ __global__ void init_identity(double* __restrict__ Z, int64_t k) {
 
 struct SvdWs {
   double *Wv, *Z, *Gpart, *Q, *sig, *S, *R, *QZ;
-  int *skip, *perm, *counter, *vp;
+  int *skip, *perm, *vp;
   unsigned long long* off;
   QrWs qr;
   int precond;
@@ -304,7 +290,6 @@ static size_t svd_layout(int64_t m, int64_t n, char* base, size_t cap, SvdWs* ou
   w.skip = a.take<int>((size_t)npairs);
   w.perm = a.take<int>((size_t)k);
   w.off = a.take<unsigned long long>(4);
-  w.counter = a.take<int>((size_t)npairs);
   if (w.precond) {
     w.R = a.take<double>((size_t)(k * k));
     w.QZ = a.take<double>((size_t)(L * k));
@@ -335,16 +320,22 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
   // algorithmic work per step: Gram 2 L (2b)^2 per pair, apply 2 (L + k) (2b)^2 per pair
   const double gram_fl = 2.0 * (double)L * J2 * J2 * npairs;
   const double apply_fl = 2.0 * (double)(L + kz) * J2 * J2 * npairs;
-  TN_HIP(hipMemsetAsync(w.counter, 0, sizeof(int) * npairs, st));
   unsigned long long host_off = 0;
   auto enqueue_sweep = [&]() -> int {
     hipLaunchKernelGGL(reset_u64, dim3(1), dim3(1), 0, st, w.off);
     for (int s = 0; s < nblk - 1; ++s) {
       {
-        // Gram partials + (last block per pair) the pair eigensolve
         ProfScope p(PROF_JAC_GRAM, st, gram_fl, 8.0 * (double)L * J2 * npairs);
         hipLaunchKernelGGL(jac_gram, dim3(npairs, nchW), dim3(256), 0, st, Wv, L, k, s, nblk,
-                           nchW, w.Gpart, w.counter, tol, w.Q, w.skip, w.off);
+                           nchW, w.Gpart);
+      }
+      {
+        // a kernel boundary between the partials and the pair eigensolve is
+        // cheaper than a device-scope fence + arrival counter in the Gram
+        // kernel (measured)
+        ProfScope p(PROF_JAC_EIG, st, 0.0, 8.0 * (double)nchW * J2 * J2 * npairs);
+        hipLaunchKernelGGL(jac_eig, dim3(npairs), dim3(256), 0, st, w.Gpart, nchW, tol, w.Q,
+                           w.skip, w.off);
       }
       {
         ProfScope p(PROF_JAC_APPLY, st, apply_fl, 16.0 * (double)(L + kz) * J2 * npairs);
@@ -518,7 +509,7 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
 //   1. P orders K by descending diagonal (a static pivot); shift
 //      delta = 8 sqrt(n) eps_in tr(K), eps_in the input's unit roundoff;
 //   2. P K P^T + delta I = L L^T, blocked right-looking Cholesky (64-column
-//      blocks: the diagonal block in one wavefront's registers, the panel
+//      blocks: the diagonal block by one wavefront in registers, the panel
 //      by row-parallel forward substitution, the trailing update by an MFMA
 //      GEMM; a column whose pivot falls below delta/2 is dropped);
 //   3. one-sided block Jacobi on the columns of L (no rotation accumulation):
@@ -567,13 +558,15 @@ __global__ void psd_permute(const double* __restrict__ A, int64_t n, const int* 
 
 // Factor the w x w diagonal block at (j0, j0) of B (column-major, ld n) and
 // write L11 (lower, zeros above) into Lw at the same position.  One
-// wavefront; lane t keeps row t of the block in registers; column c's
-// entries are exchanged through LDS (two wave-local barriers per column).
+// wavefront; lane t keeps row t of the block in registers (every loop has a
+// constant trip count, so the row is indexed statically); column c is
+// exchanged through LDS.  Padding columns (c >= w) are zero and stay zero.
 __global__ void __launch_bounds__(64) chol_diag(const double* __restrict__ B, int64_t n,
                                                 int64_t j0, int w,
                                                 const double* __restrict__ delta,
                                                 double* __restrict__ Lw) {
   __shared__ double col[CB];
+  __shared__ double piv;
   const int t = threadIdx.x;
   const double dl = *delta;
   double r[CB];
@@ -581,25 +574,23 @@ __global__ void __launch_bounds__(64) chol_diag(const double* __restrict__ B, in
   for (int c = 0; c < CB; ++c) r[c] = (t < w && c < w) ? B[(j0 + c) * n + j0 + t] : 0.0;
 #pragma unroll
   for (int c = 0; c < CB; ++c) {
-    if (c >= w) break;
-    if (t == c) col[0] = r[c];
+    if (t == c) piv = r[c];
     __syncthreads();
     // With the shift every exact pivot is >= delta; one below delta/2 means
     // the input is indefinite beyond its rounding allowance: the column is
     // dropped (zero) rather than divided by a tiny pivot.
-    const double d = col[0];
-    const double s = d >= 0.5 * dl && d > 0.0 ? sqrt(d) : 0.0;
-    const double inv = s > 0.0 ? 1.0 / s : 0.0;
-    if (t == c) r[c] = s;
+    const double d = piv;
+    const double sq = d >= 0.5 * dl && d > 0.0 ? sqrt(d) : 0.0;
+    const double inv = sq > 0.0 ? 1.0 / sq : 0.0;
+    if (t == c) r[c] = sq;
     if (t > c) r[c] *= inv;
-    __syncthreads();  // col[0] read by every lane before it is replaced
     col[t] = t > c ? r[c] : 0.0;
     __syncthreads();
     if (t > c) {
 #pragma unroll
       for (int c2 = c + 1; c2 < CB; ++c2) r[c2] -= r[c] * col[c2];
     }
-    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
   }
   if (t < w) {
 #pragma unroll
@@ -608,38 +599,36 @@ __global__ void __launch_bounds__(64) chol_diag(const double* __restrict__ B, in
   }
 }
 
-// Panel below the diagonal block: L21 = A21 L11^{-T} (forward substitution
-// per row, one thread per row, the row in registers, L11 in LDS).  Rows
+// Panel below the diagonal block: L21 = A21 L11^{-T} by forward
+// substitution, one row per lane (64 rows per block), rows and L11 in LDS;
+// each dot product runs in four independent partial sums.  Rows
 // [j0 + w, n) of columns [j0, j0 + w); A21 from B, L11 and L21 in Lw.
-__global__ void __launch_bounds__(256) chol_panel(const double* __restrict__ B, int64_t n,
-                                                  int64_t j0, int w, double* __restrict__ Lw) {
+__global__ void __launch_bounds__(64) chol_panel(const double* __restrict__ B, int64_t n,
+                                                 int64_t j0, int w, double* __restrict__ Lw) {
   __shared__ double Ls[CB][CB + 1];
+  __shared__ double xs[CB][CB + 1];
   __shared__ double rd[CB];
-  for (int e = threadIdx.x; e < CB * CB; e += 256) {
-    const int c = e / CB, rr = e % CB;  // column c, row rr of L11
-    Ls[rr][c] = (c < w && rr < w) ? Lw[(j0 + c) * n + j0 + rr] : 0.0;
-  }
+  const int t = threadIdx.x;
+  for (int c = 0; c < CB; ++c) Ls[t][c] = (c < w && t < w) ? Lw[(j0 + c) * n + j0 + t] : 0.0;
+  const int64_t i = j0 + w + (int64_t)blockIdx.x * CB + t;
+  for (int c = 0; c < CB; ++c) xs[t][c] = (c < w && i < n) ? B[(j0 + c) * n + i] : 0.0;
   __syncthreads();
-  if (threadIdx.x < CB) {
-    const double dd = Ls[threadIdx.x][threadIdx.x];
-    rd[threadIdx.x] = dd > 0.0 ? 1.0 / dd : 0.0;
-  }
+  rd[t] = Ls[t][t] > 0.0 ? 1.0 / Ls[t][t] : 0.0;
   __syncthreads();
-  const int64_t i = j0 + w + (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  double x[CB];
-#pragma unroll
-  for (int c = 0; c < CB; ++c) x[c] = c < w ? B[(j0 + c) * n + i] : 0.0;
-#pragma unroll
-  for (int c = 0; c < CB; ++c) {
-    double acc = x[c];
-#pragma unroll
-    for (int j = 0; j < c; ++j) acc -= x[j] * Ls[c][j];
-    x[c] = acc * rd[c];
+  for (int c = 0; c < w; ++c) {
+    double a0 = xs[t][c], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int j = 0;
+    for (; j + 3 < c; j += 4) {
+      a0 -= xs[t][j] * Ls[c][j];
+      a1 -= xs[t][j + 1] * Ls[c][j + 1];
+      a2 -= xs[t][j + 2] * Ls[c][j + 2];
+      a3 -= xs[t][j + 3] * Ls[c][j + 3];
+    }
+    for (; j < c; ++j) a0 -= xs[t][j] * Ls[c][j];
+    xs[t][c] = ((a0 + a1) + (a2 + a3)) * rd[c];
   }
-#pragma unroll
-  for (int c = 0; c < CB; ++c)
-    if (c < w) Lw[(j0 + c) * n + i] = x[c];
+  if (i < n)
+    for (int c = 0; c < w; ++c) Lw[(j0 + c) * n + i] = xs[t][c];
 }
 
 // U[p[i]][j] = Wv[perm[j]][i] / S[j];  lam[j] = S[j]^2 - delta
@@ -680,7 +669,6 @@ static size_t eig_layout(int64_t n, char* base, size_t cap, EigWs* out) {
   e.jw.skip = a.take<int>((size_t)npairs);
   e.jw.perm = a.take<int>((size_t)n);
   e.jw.off = a.take<unsigned long long>(4);
-  e.jw.counter = a.take<int>((size_t)npairs);
   if (out) *out = e;
   return a.used;
 }
@@ -735,8 +723,8 @@ extern "C" int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes
     const int64_t m2 = n - j0 - w;
     if (m2 <= 0) break;
     // L21 = A21 L11^{-T}   (m2 x w)
-    hipLaunchKernelGGL(chol_panel, dim3((unsigned)ceil_div(m2, 256)), dim3(256), 0, st, e.B, n, j0,
-                       w, Wv);
+    hipLaunchKernelGGL(chol_panel, dim3((unsigned)ceil_div(m2, CB)), dim3(64), 0, st, e.B, n, j0, w,
+                       Wv);
     TN_CHECK_LAUNCH();
     // A22 -= L21 L21^T
     const double* L21 = Wv + j0 * n + j0 + w;

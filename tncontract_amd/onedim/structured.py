@@ -31,8 +31,10 @@ bond then depends on the left part through its Gram matrix alone.  So:
 Precision: "fp64" runs every contraction in FP64 (the algorithmic change
 alone); "fp32" / "bf16x3" run the contractions on FP32 / split-bf16 MFMA with
 FP32 accumulation, while K, the Jacobi rotations and the output isometries
-stay FP64.  Flop count per bulk site at chi=512, D=5, d=2: ~52 GFLOP versus
-~250 GFLOP for the reference's QR + SVD on the 2560-bond chain.
+stay FP64 (the eigensolver then stops at orthogonality 1e-10 instead of
+n * eps: K formed in FP32 carries ~1e-7 relative rounding anyway).  Flop
+count per bulk site at chi=512, D=5, d=2: ~52 GFLOP versus ~250 GFLOP for
+the reference's QR + SVD on the 2560-bond chain.
 """
 from __future__ import annotations
 
@@ -53,6 +55,7 @@ class SweepStats:
         self.spectra = {}   # bond i -> sigma / sigma_0 (host, descending)
         self.kept = {}      # bond i -> kept rank
         self.avail = {}     # bond i -> number of admissible singular values
+        self.sweeps = {}    # bond i -> Jacobi sweeps of the eigensolver
 
 
 def _canon(t, labels):
@@ -65,12 +68,29 @@ def _ones(shape, dtype):
     return torch.from_numpy(host).to(L.device())
 
 
+def _reorthonormalise(Y):
+    """Y (n x k, FP64, orthonormal columns up to a small error e) ->
+    Y (3 I - Y^T Y) / 2, orthonormal up to O(e^2) (Newton-Schulz step of the
+    polar factor)."""
+    S = tsr._contract_data(Y, [0], Y, [0])                # Y^T Y
+    eye = L.diag_embed(_ones((Y.shape[1],), torch.float64))
+    X = L.axpby(S, eye, -0.5, 1.5)
+    return tsr._contract_data(Y, [1], X, [0])
+
+
 def _cdt(x, dtype):
     return L.convert(x, dtype) if x.dtype != dtype else x
 
 
+# Jacobi orthogonality tolerance of the eigensolver when K is formed from
+# FP32 / bf16x3 contractions: K itself then carries ~1e-7 relative rounding,
+# so converging the rotations to n * eps buys nothing; 1e-10 keeps the
+# eigensolver's share of the state error far below the contractions' share.
+MIXED_EIG_TOL = 1e-10
+
+
 def apply_mpo_compress(mps, mpo, chi=None, threshold=1e-15, precision="fp64", stats=None,
-                       eig_tol=0.0):
+                       eig_tol=None):
     if len(mps) != len(mpo):
         raise ValueError("MPS and MPO have different lengths")
     N = len(mps)
@@ -85,6 +105,8 @@ def apply_mpo_compress(mps, mpo, chi=None, threshold=1e-15, precision="fp64", st
         if B[i].shape[0] != W[i].shape[3]:
             raise ValueError("physical dimension mismatch at site %d" % i)
     con = lambda x, ax1, y, ax2: tsr._contract_data(x, ax1, y, ax2, prec)  # noqa: E731
+    if eig_tol is None:
+        eig_tol = 0.0 if work == torch.float64 else MIXED_EIG_TOL   # 0: n * eps
 
     # ---- 1. left environments E_i (a, alpha, a', alpha'), normalised
     E = [None] * N
@@ -115,7 +137,8 @@ def apply_mpo_compress(mps, mpo, chi=None, threshold=1e-15, precision="fp64", st
         # rows (t', k'), cols (t, k); an FP32 K goes in as FP32 so that the
         # eigensolver's shift covers its rounding
         Km = Kt.permute(1, 0, 3, 2)
-        u, lam = tsr._eigh_psd_device(Km, 2, eig_tol)
+        info = {}
+        u, lam = tsr._eigh_psd_device(Km, 2, eig_tol, info)
         lam_h = lam.cpu().numpy()
         avail = min(n, qb[i])
         sig = np.sqrt(np.maximum(lam_h[:avail], 0.0))
@@ -129,7 +152,14 @@ def apply_mpo_compress(mps, mpo, chi=None, threshold=1e-15, precision="fp64", st
             stats.spectra[i] = rel
             stats.kept[i] = keep
             stats.avail[i] = avail
-        Y = u.view(d[i], r, n)[:, :, :keep]               # (t, k, kept), FP64 isometry
+            stats.sweeps[i] = info["sweeps"]
+        if eig_tol > 0.0:
+            # the rotations stopped at orthogonality eig_tol: one Newton-Schulz
+            # step Y (3 I - Y^T Y) / 2 restores an isometry to rounding
+            Y2 = _reorthonormalise(u[:, :keep])
+            Y = Y2.view(d[i], r, keep)                    # (t, k, kept), FP64 isometry
+        else:
+            Y = u.view(d[i], r, n)[:, :, :keep]           # (t, k, kept), FP64 isometry
         sites[i] = tsr.Tensor._wrap(L.contiguous(Y), [mpo.physout_label, mps.right_label,
                                                        mps.left_label])
         P = con(C, [3, 1], _cdt(Y, work), [0, 1])         # (a, al, kept)

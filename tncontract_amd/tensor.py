@@ -557,10 +557,11 @@ def _svd_device(d, nrow):
     return u, s, vt
 
 
-def _eigh_psd_device(d, nrow, tol=0.0):
+def _eigh_psd_device(d, nrow, tol=0.0, info=None):
     """Eigendecomposition of the symmetric positive semi-definite (rows | cols)
     matrix view of d: returns (u, lam) with u: n x n float64 (column j =
-    eigenvector j) and lam: n float64, descending."""
+    eigenvector j) and lam: n float64, descending.  tol: Jacobi orthogonality
+    tolerance (0: n * eps); info (dict, optional) receives "sweeps"."""
     rows, cols, m, n = _matrix_modes(d, nrow)
     if m != n:
         raise ValueError("eigh_psd needs a square matrix, got %d x %d" % (m, n))
@@ -575,6 +576,8 @@ def _eigh_psd_device(d, nrow, tol=0.0):
                          L.ptr(lam), L.dtype_code(d), float(tol), 0, ctypes.byref(sweeps),
                          L.ptr(ws), ws.numel(), L.stream_ptr())
     L.check(rc)
+    if info is not None:
+        info["sweeps"] = sweeps.value
     return u, lam
 
 
