@@ -37,7 +37,14 @@ sys.path.insert(0, ROOT)
 METRIC = ("MPO·MPS compress sweeps/s at χ=512: mixed+learned vs plain FP64 GPU "
           "@1e-8 err")
 PEAK_TFLOPS = {"gemm_f64": 78.6, "gemm_f32": 157.3, "gemm_bf16x3": 2500.0,
-               "jac_gram": 78.6, "jac_apply": 78.6}
+               "jac_gram": 78.6, "jac_eig": 78.6, "jac_apply": 78.6}
+NOTES = {"jac_eig": "FP64 rotation arithmetic of the 32x32 pair eigensolve; latency-bound "
+                    "(31 dependent rotation rounds per launch, one barrier each)"}
+# HBM bytes per launch (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on a
+# streaming kernel) measured for the chi=512 sweep's n=1024 Jacobi launches;
+# profiles/jac_eig_chi512_pmc_traffic.md.  Used when --traffic is not given.
+MEASURED_TRAFFIC = {"jac_eig": 1274.0 * 1024, "jac_gram": 9289.0 * 1024,
+                    "jac_apply": 12590.0 * 1024}
 
 
 def parse():
@@ -52,7 +59,29 @@ def parse():
     p.add_argument("--plain-steps", type=int, default=1)
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    p.add_argument("--traffic", type=float, default=None,
+                   help="HBM bytes per launch of the dominant kernel, from the calibrated "
+                        "FETCH_SIZE + WRITE_SIZE counters (reported as roofline.traffic)")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                   help="only run a plain device copy of MB megabytes three times (the "
+                        "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters) "
+                        "and exit")
     return p.parse_args()
+
+
+def copy_calibration(mb):
+    """Three launches of the library's streaming kernel tn_scale (x *= 1 on an
+    FP64 buffer of mb MB): exactly mb MB read and mb MB written per launch."""
+    import torch
+    from tncontract_amd import _lib as L
+    torch.cuda.set_device(0)
+    n = mb * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        L.scale_(x, 1.0)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                      "launches": 3, "check": float(x[0])}), flush=True)
 
 
 def log(*a):
@@ -126,6 +155,9 @@ def cpu_baseline(N, d, chi, D=5):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every box here has one GPU); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -180,13 +212,19 @@ def main():
     if dom in PEAK_TFLOPS and fl > 0:
         ach = fl / (ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_TFLOPS[dom],
-                "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dom], "traffic": None,
-                "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n}
+                "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dom],
+                "traffic": args.traffic if args.traffic is not None else MEASURED_TRAFFIC.get(dom),
+                "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n,
+                "flops_per_launch": fl / max(n, 1)}
     else:
         ach = by / (ms * 1e-3) / 1e9
-        roof = {"bound": "hbm" if dom != "jac_eig" else "latency", "kernel": dom,
-                "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
-                "traffic": None, "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n}
+        roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": 8000.0, "unit": "GB/s",
+                "frac": ach / 8000.0,
+                "traffic": args.traffic if args.traffic is not None else MEASURED_TRAFFIC.get(dom),
+                "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n,
+                "bytes_per_launch": by / max(n, 1)}
+    if dom in NOTES:
+        roof["note"] = NOTES[dom]
     kernel_ms = {t: round(v[0], 3) for t, v in prof.items() if v[1]}
 
     # ---- plain FP64 path (the denominator) and the error between the two

@@ -333,7 +333,11 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
         // a kernel boundary between the partials and the pair eigensolve is
         // cheaper than a device-scope fence + arrival counter in the Gram
         // kernel (measured)
-        ProfScope p(PROF_JAC_EIG, st, 0.0, 8.0 * (double)nchW * J2 * J2 * npairs);
+        // algorithmic work: 31 rounds of 16 rotations applied to both sides of
+        // G and to Q, 9216 flops per round per pair; bytes: the partials read
+        // and Q written
+        ProfScope p(PROF_JAC_EIG, st, 31.0 * 9216.0 * npairs,
+                    8.0 * (double)(nchW + 1) * J2 * J2 * npairs);
         hipLaunchKernelGGL(jac_eig, dim3(npairs), dim3(256), 0, st, w.Gpart, nchW, tol, w.Q,
                            w.skip, w.off);
       }
