@@ -37,14 +37,13 @@ sys.path.insert(0, ROOT)
 METRIC = ("MPO·MPS compress sweeps/s at χ=512: mixed+learned vs plain FP64 GPU "
           "@1e-8 err")
 PEAK_TFLOPS = {"gemm_f64": 78.6, "gemm_f32": 157.3, "gemm_bf16x3": 2500.0,
-               "jac_gram": 78.6, "jac_eig": 78.6, "jac_apply": 78.6}
-NOTES = {"jac_eig": "FP64 rotation arithmetic of the 32x32 pair eigensolve; latency-bound "
-                    "(31 dependent rotation rounds per launch, one barrier each)"}
+               "jac_gram": 78.6, "jac_eig_apply": 78.6, "jac_apply": 78.6}
+NOTES = {"jac_eig_apply": "pair eigensolve (FP64 rotations, 31 dependent rounds with one barrier "
+                          "each) + MFMA apply; latency-bound"}
 # HBM bytes per launch (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on a
 # streaming kernel) measured for the chi=512 sweep's n=1024 Jacobi launches;
 # profiles/jac_eig_chi512_pmc_traffic.md.  Used when --traffic is not given.
-MEASURED_TRAFFIC = {"jac_eig": 1274.0 * 1024, "jac_gram": 9289.0 * 1024,
-                    "jac_apply": 12590.0 * 1024}
+MEASURED_TRAFFIC = {}
 
 
 def parse():

@@ -188,7 +188,7 @@ int tn_randn(void* out, int dtype, int64_t n, unsigned long long seed, void* str
  * synchronises those events and returns, for one tag, the summed elapsed
  * time, launch count and the algorithmic flops / bytes the launcher assigned
  * to each launch.  Tags: 0 FP64 GEMM, 1 FP32 GEMM, 2 BF16x3 GEMM, 3 Jacobi
- * Gram, 4 Jacobi pair eigensolve, 5 Jacobi apply, 6 QR panel, 7 direct
+ * Gram, 4 Jacobi pair eigensolve (+ apply in tn_svd), 5 Jacobi apply, 6 QR panel, 7 direct
  * (small-K) contraction. */
 int tn_prof_enable(int on);
 int tn_prof_collect(int tag, double* total_ms, int64_t* launches, double* flops, double* bytes,

@@ -113,7 +113,7 @@ def check(rc):
     raise RuntimeError(msg)
 
 
-PROF_TAGS = {"gemm_f64": 0, "gemm_f32": 1, "gemm_bf16x3": 2, "jac_gram": 3, "jac_eig": 4,
+PROF_TAGS = {"gemm_f64": 0, "gemm_f32": 1, "gemm_bf16x3": 2, "jac_gram": 3, "jac_eig_apply": 4,
              "jac_apply": 5, "qr_panel": 6, "direct": 7}
 
 

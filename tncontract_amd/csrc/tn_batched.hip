@@ -110,6 +110,7 @@ __global__ void __launch_bounds__(256) gemm_grouped_f32(const GJob* __restrict__
 // ---------------------------------------------------------------- batched Jacobi
 constexpr int BJB = 16, BJ2 = 32, BCH = 256;
 static_assert(BJB == JB && BJ2 == J2, "batched pairs use the shared pair layout");
+static_assert(BCH == 256, "apply_mfma covers 256 rows per block");
 
 struct BJob {
   int64_t wv;     // offset of the job's vectors (k vectors of length L)

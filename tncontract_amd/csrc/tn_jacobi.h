@@ -28,6 +28,16 @@ __device__ __forceinline__ int64_t pair_vec(int c, int I, int J) {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// fences global memory, which on gfx9 means waiting for every outstanding
+// global load (one counter for loads and stores); this barrier lets loads
+// issued before it stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 struct EigLds {
   double G[2][J2][J2 + 1];
   double Q[2][J2][J2 + 1];
@@ -52,9 +62,9 @@ __device__ inline double eig_off(EigLds& sm) {
   }
   off = wave_max(off);
   if ((tid & 63) == 0) sm.red[tid >> 6] = off;
-  __syncthreads();
+  lds_barrier();
   off = fmax(fmax(sm.red[0], sm.red[1]), fmax(sm.red[2], sm.red[3]));
-  __syncthreads();
+  lds_barrier();
   return off;
 }
 
@@ -145,7 +155,7 @@ __device__ inline int eig_rotate(EigLds& sm, double tol) {
     x1 = x1 == J2 - 1 ? 1 : x1 + 1;
     if (y0 != 0) y0 = y0 == J2 - 1 ? 1 : y0 + 1;
     y1 = y1 == J2 - 1 ? 1 : y1 + 1;
-    __syncthreads();
+    lds_barrier();
   }
   return cur;
 }
@@ -153,64 +163,83 @@ __device__ inline int eig_rotate(EigLds& sm, double tol) {
 // Per-wave LDS staging tiles of apply_mfma (64 rows x 32 vectors per wave).
 typedef double ApplyTile[4][64][J2 + 1];
 
-// Rows [rs, re) of the pair's 32 vectors (vector v at base + v * ld, nv
-// vectors) <- rows . Q, Q 32 x 32 (row-major Qg in global memory, staged in
-// the LDS buffer Q, row stride J2 + 1).  Each wave
-// takes 64 rows per pass: it loads the 64 x 32 tile column by column (64
-// consecutive rows per load) into its LDS tile, multiplies it by Q with
-// v_mfma_f64_16x16x4 (A operands from the tile, B from Q), writes the
-// product back into the tile and stores it column by column.  The pass
-// count is the same for every wave (block-wide barriers), 256 threads.
-__device__ inline void apply_mfma(double* __restrict__ base, int64_t ld, int64_t nv, int64_t rs,
-                                  int64_t re, int I, int J, const double* __restrict__ Qg,
-                                  double (*Q)[J2 + 1], ApplyTile& Tw) {
+// Rows [rs, re) (re - rs <= 256) of the pair's 32 vectors (vector v at
+// base + v * ld, nv vectors) <- rows . Q, in two halves so that a caller can
+// put other work between them while the loads are in flight:
+//   apply_tile_fetch:  thread t (wave w, lane l) loads row rs + 64 w + l of
+//                      the 32 vectors into registers (64 consecutive rows per
+//                      load across a wave);
+//   apply_tile_finish: stores the rows into the wave's LDS tile, multiplies
+//                      the tiles by Q (32 x 32 in LDS, row stride J2 + 1)
+//                      with v_mfma_f64_16x16x4 (A operands from the tile, B
+//                      from Q), writes the product back into the tile and
+//                      stores it column by column.
+// 256 threads; LDS-only barriers inside apply_tile_finish.
+__device__ __forceinline__ void apply_tile_fetch(const double* __restrict__ base, int64_t ld,
+                                                 int64_t nv, int64_t rs, int64_t re, int I,
+                                                 int J, double (&x)[J2]) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r = rs + 64 * wave + lane;
+#pragma unroll
+  for (int c = 0; c < J2; ++c) {
+    const int64_t v = pair_vec(c, I, J);
+    x[c] = (v < nv && r < re) ? base[v * ld + r] : 0.0;
+  }
+}
+
+__device__ inline void apply_tile_finish(double* __restrict__ base, int64_t ld, int64_t nv,
+                                         int64_t rs, int64_t re, int I, int J,
+                                         const double (&x)[J2], const double (*Q)[J2 + 1],
+                                         ApplyTile& Tw) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
   auto& T = Tw[wave];
-  for (int64_t rb = rs; rb < re; rb += 256) {
-    const int64_t r0 = rb + 64 * wave;
-    const int64_t r = r0 + lane;
+  const int64_t r = rs + 64 * wave + lane;
 #pragma unroll
-    for (int c = 0; c < J2; ++c) {
-      const int64_t v = pair_vec(c, I, J);
-      T[lane][c] = (v < nv && r < re) ? base[v * ld + r] : 0.0;
-    }
-    if (rb == rs)  // Q (global, 32 x 32) -> LDS, its loads in flight with the tile's
-      for (int e = threadIdx.x; e < J2 * J2; e += 256) Q[e / J2][e % J2] = Qg[e];
-    __syncthreads();
-    double a[4][8];
+  for (int c = 0; c < J2; ++c) T[lane][c] = x[c];
+  lds_barrier();
+  double a[4][8];
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks)
+  for (int ks = 0; ks < 8; ++ks)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) a[t][ks] = T[16 * t + lr][4 * ks + lk];
-    f64x4 acc[4][2];
+    for (int t = 0; t < 4; ++t) a[t][ks] = T[16 * t + lr][4 * ks + lk];
+  f64x4 acc[4][2];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = f64x4{0, 0, 0, 0};
+  for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = f64x4{0, 0, 0, 0};
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
+  for (int ks = 0; ks < 8; ++ks) {
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
-        const double b = Q[4 * ks + lk][16 * ct + lr];
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-          acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][ks], b, acc[t][ct], 0, 0, 0);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
+    for (int ct = 0; ct < 2; ++ct) {
+      const double b = Q[4 * ks + lk][16 * ct + lr];
 #pragma unroll
       for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) T[16 * t + lk + 4 * e][16 * ct + lr] = acc[t][ct][e];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < J2; ++c) {
-      const int64_t v = pair_vec(c, I, J);
-      if (v < nv && r < re) base[v * ld + r] = T[lane][c];
+        acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][ks], b, acc[t][ct], 0, 0, 0);
     }
-    __syncthreads();
   }
+  lds_barrier();
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) T[16 * t + lk + 4 * e][16 * ct + lr] = acc[t][ct][e];
+  lds_barrier();
+#pragma unroll
+  for (int c = 0; c < J2; ++c) {
+    const int64_t v = pair_vec(c, I, J);
+    if (v < nv && r < re) base[v * ld + r] = T[lane][c];
+  }
+}
+
+// apply_tile_fetch + (Q from the row-major Qg in global memory into the LDS
+// buffer Q) + apply_tile_finish.
+__device__ inline void apply_mfma(double* __restrict__ base, int64_t ld, int64_t nv, int64_t rs,
+                                  int64_t re, int I, int J, const double* __restrict__ Qg,
+                                  double (*Q)[J2 + 1], ApplyTile& Tw) {
+  double x[J2];
+  apply_tile_fetch(base, ld, nv, rs, re, I, J, x);
+  for (int e = threadIdx.x; e < J2 * J2; e += 256) Q[e / J2][e % J2] = Qg[e];
+  apply_tile_finish(base, ld, nv, rs, re, I, J, x, Q, Tw);
 }
 
 }  // namespace tn

@@ -9,14 +9,14 @@
 // for every pair (I,J) at once:
 //   1. jac_gram:  partial Gram G = P^T P of P = [W_I W_J] over a row chunk,
 //                 v_mfma_f64_16x16x4 from an LDS tile (grid: pairs x chunks);
-//   2. jac_eig:   sums the partials, records the pair's largest normalised
-//                 off-diagonal |G_pq|/sqrt(G_pp G_qq), and runs one cyclic
-//                 sweep of two-sided Jacobi on the 32x32 G in LDS,
-//                 accumulating Q (grid: pairs);
-//   3. jac_apply: [W_I W_J] <- [W_I W_J] Q and [Z_I Z_J] <- [Z_I Z_J] Q by
-//                 MFMA (grid: pairs x row chunks).
-// The three kernels of every step of a sweep are captured once as a HIP
-// graph and replayed.
+//   2. jac_eig_apply: every block of a pair sums the partials, records the
+//                 largest normalised off-diagonal |G_pq|/sqrt(G_pp G_qq),
+//                 runs one cyclic sweep of two-sided Jacobi on the 32x32 G
+//                 in LDS accumulating Q (the same Q in every block), and
+//                 applies [W_I W_J] <- [W_I W_J] Q (or [Z_I Z_J] Q) to its
+//                 row chunk by MFMA (grid: pairs x row chunks).
+// The two kernels of every step of a sweep are captured once as a HIP graph
+// and replayed.
 // A sweep is nblk-1 steps; sweeps repeat until the largest normalised
 // off-diagonal of a whole sweep is <= tol.  Singular values are the vector
 // norms; they are sorted descending on the device.
@@ -31,7 +31,8 @@
 
 namespace tn {
 
-constexpr int JCH = 256;     // rows per chunk (jac_gram / jac_apply)
+constexpr int JCH = 256;     // rows per chunk (jac_gram / jac_eig_apply)
+static_assert(JCH == 256, "apply_tile_load/finish cover 256 rows per block");
 
 // Partial Gram P^T P of the pair's 32 vectors over rows [r0, r1) into out
 // (32 x 32, row stride ostride; global or LDS), v_mfma_f64_16x16x4 from a
@@ -65,7 +66,7 @@ __device__ void gram_partial(double (*Ps)[64][J2 + 1], const double* __restrict_
     load(r0);
     store(0);
   }
-  __syncthreads();
+  lds_barrier();
   for (int sb = 0; sb < nsub; ++sb) {
     const int buf = sb & 1;
     if (sb + 1 < nsub) load(r0 + (int64_t)(sb + 1) * 64);
@@ -76,7 +77,7 @@ __device__ void gram_partial(double (*Ps)[64][J2 + 1], const double* __restrict_
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
     if (sb + 1 < nsub) store(buf ^ 1);
-    __syncthreads();
+    lds_barrier();
   }
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -94,26 +95,7 @@ __device__ void eig_load(EigLds& sm, const double* __restrict__ Gp, int nchunk) 
     sm.G[0][e / J2][e % J2] = acc;
     sm.Q[0][e / J2][e % J2] = (e / J2 == e % J2) ? 1.0 : 0.0;
   }
-  __syncthreads();
-}
-
-// Pair eigensolve from the partials: records the normalised off-diagonal,
-// diagonalises G = Q diag Q^T in LDS and writes Q (or the skip flag).
-__device__ void pair_eig(EigLds& sm, const double* __restrict__ Gpart, int nchunk, int pi,
-                         double tol, double* __restrict__ Qall, int* __restrict__ skip,
-                         unsigned long long* __restrict__ offmax) {
-  const int tid = threadIdx.x;
-  eig_load(sm, Gpart + (int64_t)pi * nchunk * (J2 * J2), nchunk);
-  const double off = eig_off(sm);
-  if (tid == 0) atomicMax(offmax, dbits(off));
-  if (off <= tol) {
-    if (tid == 0) skip[pi] = 1;
-    return;
-  }
-  if (tid == 0) skip[pi] = 0;
-  const int cur = eig_rotate(sm, tol);
-  double* out = Qall + (int64_t)pi * (J2 * J2);
-  for (int e = tid; e < J2 * J2; e += 256) out[e] = sm.Q[cur][e / J2][e % J2];
+  lds_barrier();
 }
 
 __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, int64_t L,
@@ -128,33 +110,38 @@ __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, i
   gram_partial(Ps, Wv, L, nv, I, J, r0, r1, Gpart + ((int64_t)pi * nchunk + ch) * (J2 * J2));
 }
 
-__global__ void __launch_bounds__(256) jac_eig(const double* __restrict__ Gpart, int nchunk,
-                                               double tol, double* __restrict__ Qall,
-                                               int* __restrict__ skip,
-                                               unsigned long long* __restrict__ offmax) {
+// Pair eigensolve + apply, grid pairs x (row chunks of Wv, then of Z):
+// every block of a pair sums the pair's Gram partials in the same order,
+// records the normalised off-diagonal (chunk 0 only), runs the same inner
+// Jacobi sweep in LDS (bit-identical Q in all of them) and applies Q to its
+// own row chunk by MFMA.  Computing the 32 x 32 eigensolve redundantly per
+// chunk costs nothing in latency and saves a launch and the Q round trip
+// through memory.
+__global__ void __launch_bounds__(256) jac_eig_apply(double* __restrict__ Wv, int64_t L,
+                                                     double* __restrict__ Z, int64_t nv, int s,
+                                                     int nblk, int nchW,
+                                                     const double* __restrict__ Gpart, double tol,
+                                                     unsigned long long* __restrict__ offmax) {
   __shared__ EigLds sm;
-  pair_eig(sm, Gpart, nchunk, blockIdx.x, tol, Qall, skip, offmax);
-}
-
-// rows [ch*JCH, ...) of the stacked [Wv (L rows); Z (nv rows)] for pair pi
-__global__ void __launch_bounds__(256) jac_apply(double* __restrict__ Wv, int64_t L,
-                                                 double* __restrict__ Z, int64_t nv, int s,
-                                                 int nblk, int nchW, const double* __restrict__ Qall,
-                                                 const int* __restrict__ skip) {
-  __shared__ double Q[J2][J2 + 1];
   __shared__ ApplyTile Tw;
   const int pi = blockIdx.x, ch = blockIdx.y;
-  if (skip[pi]) return;
   int I, J;
   pair_blocks(pi, s, nblk, I, J);
-  const double* Qg = Qall + (int64_t)pi * J2 * J2;
+  double* base;
+  int64_t ld, rs, re;
   if (ch < nchW) {
-    const int64_t r = (int64_t)ch * JCH;
-    apply_mfma(Wv, L, nv, r, min(L, r + JCH), I, J, Qg, Q, Tw);
+    base = Wv; ld = L; rs = (int64_t)ch * JCH; re = min(L, rs + JCH);
   } else {
-    const int64_t r = (int64_t)(ch - nchW) * JCH;
-    apply_mfma(Z, nv, nv, r, min(nv, r + JCH), I, J, Qg, Q, Tw);
+    base = Z; ld = nv; rs = (int64_t)(ch - nchW) * JCH; re = min(nv, rs + JCH);
   }
+  eig_load(sm, Gpart + (int64_t)pi * nchW * (J2 * J2), nchW);
+  double x[J2];
+  apply_tile_fetch(base, ld, nv, rs, re, I, J, x);   // in flight during the rounds
+  const double off = eig_off(sm);
+  if (ch == 0 && threadIdx.x == 0) atomicMax(offmax, dbits(off));
+  if (off <= tol) return;
+  const int cur = eig_rotate(sm, tol);
+  apply_tile_finish(base, ld, nv, rs, re, I, J, x, sm.Q[cur], Tw);
 }
 
 // sigma_j = ||Wv_j|| (one block per vector)
@@ -263,7 +250,7 @@ __global__ void init_identity(double* __restrict__ Z, int64_t k) {
 }
 
 struct SvdWs {
-  double *Wv, *Z, *Gpart, *Q, *sig, *S, *R, *QZ;
+  double *Wv, *Z, *Gpart, *sig, *S, *R, *QZ;
   int *skip, *perm, *vp;
   unsigned long long* off;
   QrWs qr;
@@ -284,7 +271,6 @@ static size_t svd_layout(int64_t m, int64_t n, char* base, size_t cap, SvdWs* ou
   w.Wv = a.take<double>((size_t)(L * k));
   w.Z = a.take<double>((size_t)(k * k));
   w.Gpart = a.take<double>((size_t)npairs * ceil_div(Lj, JCH) * J2 * J2);
-  w.Q = a.take<double>((size_t)npairs * J2 * J2);
   w.sig = a.take<double>((size_t)k);
   w.S = a.take<double>((size_t)k);
   w.skip = a.take<int>((size_t)npairs);
@@ -332,19 +318,13 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
       {
         // a kernel boundary between the partials and the pair eigensolve is
         // cheaper than a device-scope fence + arrival counter in the Gram
-        // kernel (measured)
-        // algorithmic work: 31 rounds of 16 rotations applied to both sides of
-        // G and to Q, 9216 flops per round per pair; bytes: the partials read
-        // and Q written
-        ProfScope p(PROF_JAC_EIG, st, 31.0 * 9216.0 * npairs,
-                    8.0 * (double)(nchW + 1) * J2 * J2 * npairs);
-        hipLaunchKernelGGL(jac_eig, dim3(npairs), dim3(256), 0, st, w.Gpart, nchW, tol, w.Q,
-                           w.skip, w.off);
-      }
-      {
-        ProfScope p(PROF_JAC_APPLY, st, apply_fl, 16.0 * (double)(L + kz) * J2 * npairs);
-        hipLaunchKernelGGL(jac_apply, dim3(npairs, nchW + nchZ), dim3(256), 0, st, Wv, L, Z, k,
-                           s, nblk, nchW, w.Q, w.skip);
+        // kernel (measured).  Work: the inner sweep (31 rounds x 9216 flops)
+        // once per pair plus the apply; bytes: partials per block + chunks
+        ProfScope p(PROF_JAC_EIG, st, 31.0 * 9216.0 * npairs + apply_fl,
+                    8.0 * (double)nchW * J2 * J2 * npairs * (nchW + nchZ) +
+                        16.0 * (double)(L + kz) * J2 * npairs);
+        hipLaunchKernelGGL(jac_eig_apply, dim3(npairs, nchW + nchZ), dim3(256), 0, st, Wv, L, Z,
+                           k, s, nblk, nchW, w.Gpart, tol, w.off);
       }
     }
     TN_CHECK_LAUNCH();
@@ -667,7 +647,6 @@ static size_t eig_layout(int64_t n, char* base, size_t cap, EigWs* out) {
   e.delta = a.take<double>(4);
   e.p = a.take<int>((size_t)n);
   e.jw.Gpart = a.take<double>((size_t)npairs * ceil_div(n, JCH) * J2 * J2);
-  e.jw.Q = a.take<double>((size_t)npairs * J2 * J2);
   e.jw.sig = a.take<double>((size_t)n);
   e.jw.S = a.take<double>((size_t)n);
   e.jw.skip = a.take<int>((size_t)npairs);
