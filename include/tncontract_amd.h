@@ -148,7 +148,9 @@ int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes* k_cols, i
  * batched one-sided block Jacobi with FP64 rotations).
  * C_j = L_j R_j (m x n); C_j = U S V^T (thin, sorted descending).  kept_out[j]
  * (host) = #{sigma/sigma_0 > rel_threshold}, capped at chi_max (>0).  The
- * first kept columns of U / rows of Vt are the truncated factors. */
+ * first kept columns of U / rows of Vt are the truncated factors.  The Jacobi
+ * stops at orthogonality max(1e-10, L eps) (L the vector length): C_j is
+ * formed in FP32, so its singular vectors are defined to ~1e-7 anyway. */
 typedef struct tn_trunc_job {
   const void* L; /* m x k FP32 row-major */
   const void* R; /* k x n FP32 row-major */

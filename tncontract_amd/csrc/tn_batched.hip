@@ -481,7 +481,11 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
     b.nchz = (int)ceil_div(k, BCH);
     b.gram_base = (int)g_job.size();
     b.eig_base = (int)e_job.size();
-    b.tol = std::max(1e-15, (double)L * 2.220446049250313e-16);
+    // The jobs' products are formed from FP32 factors with FP32 accumulation
+    // (~1e-7 relative rounding): orthogonality beyond 1e-10 is below the
+    // data's precision, and orthogonalising its noise-level vectors to
+    // L * eps costs up to three times the sweeps.
+    b.tol = std::max(1e-10, (double)L * 2.220446049250313e-16);
     for (int p = 0; p < b.nblk / 2; ++p) {
       for (int c = 0; c < b.nch; ++c) { g_job.push_back(j); g_pair.push_back(p); g_chunk.push_back(c); }
       e_job.push_back(j);
