@@ -378,6 +378,18 @@ def tensor_qr(tensor, row_labels, qr_label="qr_"):
     return Q, R
 
 
+def tensor_lq(tensor, row_labels, lq_label="lq_"):
+    """tensor.py:1058-1109 -- a QR of the column labels, relabelled: returns
+    (L, Q) with L labels row_labels + [lq_label + "in"], Q labels
+    [lq_label + "out"] + the remaining labels."""
+    col_labels = [x for x in tensor.labels if x not in row_labels]
+    tmp = unique_label()
+    Q, L = tensor_qr(tensor, col_labels, qr_label=tmp)
+    Q.replace_label(tmp + "in", lq_label + "out")
+    L.replace_label(tmp + "out", lq_label + "in")
+    return L, Q
+
+
 def con(*args):
     """tncon.py:4-159 -- internal traces first, then pairwise contractions in
     argument order (result stored at the lower tensor index), then the tensor
@@ -642,6 +654,82 @@ class OMPS(_O1D):
         return first_not_left, first_not_right
 
 
+    def variational_compress(self, chi, max_iter=10, initial_guess=None, tolerance=1e-15,
+                             normalise=False):
+        """onedim_core.py:486-631 -- start from svd_compress(chi, reverse=True)
+        (or the left-canonised initial guess), then alternate right-to-left
+        and (mirrored) left-to-right single-site sweeps: each site is the
+        contraction of the target with the left/right environments,
+        right-canonised by an LQ decomposition whose L moves to the
+        neighbour.  Converged when the relative changes of the norms along
+        the second sweep are all below `tolerance`; returns a new MPS."""
+        if initial_guess is None:
+            mps = self.copy()
+            mps.svd_compress(chi=chi, reverse=True)
+        else:
+            mps = initial_guess
+            mps.left_canonise(qr_decomposition=True)
+        mps.replace_labels([mps.left_label, mps.right_label, mps.phys_label],
+                           [unique_label(), unique_label(), unique_label()])
+        le_label = unique_label()
+        left_env = ladder_contract(mps, self, mps.phys_label, self.phys_label,
+                                   return_intermediate_contractions=True,
+                                   right_output_label=le_label, complex_conjugate_array1=True)
+        target = self
+
+        def sweep(mps1, mps2, left_env):
+            le = left_env[0].labels[0][:-1]
+            re = unique_label()
+            lq = unique_label()
+            right_envs = []
+            norms = [mps1[-1].norm()]
+            renv = None
+            for i in range(mps2.nsites - 1, 0, -1):
+                upd = contract(mps2[i], left_env[i - 1], mps2.left_label, le + "2")
+                if i != mps2.nsites - 1:
+                    upd = contract(upd, renv, mps2.right_label, re + "2")
+                    upd.replace_label(re + "1", mps1.right_label)
+                upd.replace_label([le + "1", mps2.phys_label], [mps1.left_label, mps1.phys_label])
+                Lt, Q = tensor_lq(upd, mps1.left_label, lq_label=lq)
+                Q.replace_label(lq + "out", mps1.left_label)
+                Lt.replace_label(lq + "in", mps1.right_label)
+                mps1[i] = Q
+                mps1[i - 1] = contract(mps1[i - 1], Lt, mps1.right_label, mps1.left_label)
+                norms.append(mps1[i - 1].norm())
+                if i == mps2.nsites - 1:
+                    renv = contract(conjugate(mps1[i]), mps2[i], mps1.phys_label,
+                                    target.phys_label)
+                    renv.remove_all_dummy_indices(labels=[mps1.right_label, mps2.right_label])
+                else:
+                    renv.contract(conjugate(mps1[i]), re + "1", mps1.right_label)
+                    renv.contract(mps2[i], [mps1.phys_label, re + "2"],
+                                  [target.phys_label, target.right_label])
+                renv.replace_label([mps1.left_label, mps2.left_label], [re + "1", re + "2"])
+                right_envs.append(renv.copy())
+                if i == 1:
+                    upd = contract(mps2[0], renv, mps2.right_label, re + "2")
+                    upd.replace_label([mps2.phys_label, re + "1"],
+                                      [mps1.phys_label, mps1.right_label])
+                    mps1[0] = upd
+            return right_envs, np.array(norms)
+
+        for it in range(max_iter):
+            left_env, norms1 = sweep(mps, self, left_env)
+            mps.reverse()
+            self.reverse()
+            left_env, norms2 = sweep(mps, self, left_env)
+            mps.reverse()
+            self.reverse()
+            if np.all(np.abs(norms2[1:] - norms2[:-1]) / norms2[1:] < tolerance):
+                mps.replace_labels([mps.left_label, mps.right_label, mps.phys_label],
+                                   [self.left_label, self.right_label, self.phys_label])
+                if normalise:
+                    mps[-1].data = mps[-1].data / mps.norm(canonical_form="left")
+                return mps
+            elif it == max_iter - 1:
+                raise RuntimeError("variational_compress did not converge.")
+
+
 class OMPO(_O1D):
     """onedim_core.py:1329-1367"""
 
@@ -663,8 +751,17 @@ def contract_mps_mpo(mps, mpo):
     return OMPS(out, mps.left_label, mps.right_label, mpo.physout_label)
 
 
-def ladder_contract(array1, array2, label1, label2, complex_conjugate_array1=False):
-    """onedim_core.py:1491-1663 (full chain, left-to-right branch)."""
+def ladder_contract(array1, array2, label1, label2, start=0, end=None,
+                    complex_conjugate_array1=False, left_output_label="left",
+                    right_output_label="right", return_intermediate_contractions=False):
+    """onedim_core.py:1491-1663 -- all three branches: from the left when the
+    interval starts at site 0, from the right when it ends at both arrays'
+    last site, pairwise otherwise; optionally the list of partial
+    contractions (in site order)."""
+    if end is None:
+        end = min(array1.nsites, array2.nsites) - 1
+    if end < start:
+        raise ValueError("Badly defined interval (end before start).")
     a1, a2 = array1.copy(), array2.copy()
     if complex_conjugate_array1:
         a1.complex_conjugate()
@@ -673,14 +770,55 @@ def ladder_contract(array1, array2, label1, label2, complex_conjugate_array1=Fal
     rung = unique_label()
     a1.replace_labels(label1, rung)
     a2.replace_labels(label2, rung)
-    end = min(a1.nsites, a2.nsites) - 1
-    C = contract(a1[0], a2[0], rung, rung)
-    for i in range(1, end + 1):
-        C.contract(a1[i], a1.right_label, a1.left_label)
-        C.contract(a2[i], [a2.right_label, rung], [a2.left_label, rung])
-    C.replace_label([a1.right_label, a2.right_label], ["right1", "right2"])
-    C.remove_all_dummy_indices()
-    return C
+    r1, r2 = right_output_label + "1", right_output_label + "2"
+    l1, l2 = left_output_label + "1", left_output_label + "2"
+    inter = []
+    if start == 0:
+        for i in range(0, end + 1):
+            if i == 0:
+                C = contract(a1[0], a2[0], rung, rung)
+            else:
+                C.contract(a1[i], a1.right_label, a1.left_label)
+                C.contract(a2[i], [a2.right_label, rung], [a2.left_label, rung])
+            if return_intermediate_contractions:
+                t = C.copy()
+                t.replace_label([a1.right_label, a2.right_label], [r1, r2])
+                t.remove_all_dummy_indices(labels=[x for x in t.labels if x not in [r1, r2]])
+                inter.append(t)
+        C.replace_label([a1.right_label, a2.right_label], [r1, r2])
+        C.remove_all_dummy_indices()
+    elif end == a1.nsites - 1 and end == a2.nsites - 1:
+        for i in range(end, start - 1, -1):
+            if i == end:
+                C = contract(a1[end], a2[end], rung, rung)
+            else:
+                C.contract(a1[i], a1.left_label, a1.right_label)
+                C.contract(a2[i], [a2.left_label, rung], [a2.right_label, rung])
+            if return_intermediate_contractions:
+                t = C.copy()
+                t.replace_label([a1.left_label, a2.left_label], [l1, l2])
+                t.remove_all_dummy_indices(labels=[x for x in t.labels if x not in [l1, l2]])
+                inter.insert(0, t)
+        C.replace_label([a1.left_label, a2.left_label], [l1, l2])
+        C.remove_all_dummy_indices()
+    else:
+        for i in range(start, end + 1):
+            t = contract(a1[i], a2[i], rung, rung)
+            if i == start:
+                C = t
+            else:
+                C.contract(t, [a1.right_label, a2.right_label], [a1.left_label, a2.left_label])
+            if return_intermediate_contractions:
+                t = C.copy()
+                t.replace_label([a1.right_label, a2.right_label, a1.left_label, a2.left_label],
+                                [r1, r2, l1, l2])
+                t.remove_all_dummy_indices(labels=[x for x in t.labels if x not in [r1, r2, l1, l2]])
+                t.remove_all_dummy_indices()
+                inter.append(t)
+        C.replace_label([a1.right_label, a2.right_label, a1.left_label, a2.left_label],
+                        [r1, r2, l1, l2])
+        C.remove_all_dummy_indices()
+    return inter if return_intermediate_contractions else C
 
 
 def inner_product_mps(mps_bra, mps_ket, complex_conjugate_bra=True, return_whole_tensor=False):
@@ -714,6 +852,19 @@ def svd_compress_mps(orig_mps, chi, threshold=1e-15, normalise=False):
     """onedim_core.py:1468-1473"""
     m = left_canonical_form_mps(orig_mps, threshold=threshold, normalise=normalise)
     return right_canonical_form_mps(m, chi=chi, threshold=threshold, normalise=normalise)
+
+
+def variational_compress_mps(mps, chi, max_iter=10, initial_guess=None, tolerance=1e-15):
+    """onedim_core.py:1474-1478"""
+    return mps.variational_compress(chi, max_iter=max_iter, initial_guess=initial_guess,
+                                    tolerance=tolerance)
+
+
+def conjugate(tensor):
+    """tensor.py:1185-1189"""
+    t = tensor.copy()
+    t.conjugate()
+    return t
 
 
 def expvals_mps(mps, oplist=[], sites=None, output_label=None, canonised=None):

@@ -285,3 +285,114 @@ def test_reference_fixture_invariants(fixture_mps_arrays):
     np.testing.assert_almost_equal(psi.norm(), 1.0, decimal=10)
     psi2 = fixture_omps(arrays, meta)
     np.testing.assert_almost_equal(o.inner_product_mps(psi, psi2) ** 2 / n2, 1.0, decimal=10)
+
+
+# ---------------------------------------------------------------- LQ, ladders, variational
+def test_tensor_lq_reconstruction_and_triangular(rng):
+    t = T((3, 4, 5), ["a", "b", "c"], rng)
+    Lt, Q = o.tensor_lq(t, ["a", "b"], lq_label="z")
+    # L is the R factor of a QR of the (c | a b) matrix, Q its Q factor
+    assert Lt.labels == ["zin", "a", "b"] and Q.labels == ["c", "zout"]
+    l = Lt.data.reshape(-1, 12).T
+    q = Q.data.T
+    # rows of q orthonormal, l lower triangular, l q == the (a b | c) matrix
+    np.testing.assert_allclose(q @ q.T, np.eye(q.shape[0]), atol=1e-13)
+    assert np.allclose(np.triu(l[: l.shape[1]], 1), 0.0)
+    np.testing.assert_allclose(l @ q, t.data.reshape(12, 5), atol=1e-13)
+    Lt2, Q2 = o.tensor_lq(t, ["c"], lq_label="z")
+    np.testing.assert_allclose(_labelled(o.contract(Lt2, Q2, "zin", "zout"), ["a", "b", "c"]),
+                               t.data, atol=1e-13)
+
+
+def _transfer_chain(a, b, lo, hi):
+    """Dense product of the bra/ket transfer matrices of sites lo..hi:
+    E[l1, l2, r1, r2] (site arrays ordered (phys, left, right))."""
+    E = None
+    for i in range(lo, hi + 1):
+        A = om._site_array(a[i], a.phys_label, a.left_label, a.right_label)
+        B = om._site_array(b[i], b.phys_label, b.left_label, b.right_label)
+        e = np.einsum("apb,cpd->acbd", A, B)
+        E = e if E is None else np.einsum("acbd,bdef->acef", E, e)
+    return E
+
+
+def _labelled(t, labels):
+    return np.transpose(t.data, [t.labels.index(x) for x in labels])
+
+
+def test_ladder_contract_branches_match_transfer_products(rng):
+    N = 6
+    a = random_omps(N, 2, 3, rng)
+    b = random_omps(N, 2, 4, rng)
+    full = _transfer_chain(a, b, 0, N - 1)[0, 0, 0, 0]
+    np.testing.assert_allclose(float(o.ladder_contract(a, b, "phys", "phys").data), full, rtol=1e-12)
+    # left branch, partial end, intermediates
+    inter = o.ladder_contract(a, b, "phys", "phys", end=3, return_intermediate_contractions=True)
+    assert len(inter) == 4
+    for k, t in enumerate(inter):
+        np.testing.assert_allclose(_labelled(t, ["right1", "right2"]),
+                                   _transfer_chain(a, b, 0, k)[0, 0], rtol=1e-12, atol=1e-12)
+    allint = o.ladder_contract(a, b, "phys", "phys", return_intermediate_contractions=True)
+    np.testing.assert_allclose(allint[-1].data.item(), full, rtol=1e-12)
+    # right branch (interval ends at the last site)
+    r = o.ladder_contract(a, b, "phys", "phys", start=2, end=N - 1)
+    np.testing.assert_allclose(_labelled(r, ["left1", "left2"]),
+                               _transfer_chain(a, b, 2, N - 1)[:, :, 0, 0], rtol=1e-12, atol=1e-12)
+    rint = o.ladder_contract(a, b, "phys", "phys", start=2, end=N - 1,
+                             return_intermediate_contractions=True)
+    for k, t in zip(range(2, N), rint):
+        np.testing.assert_allclose(_labelled(t, ["left1", "left2"]),
+                                   _transfer_chain(a, b, k, N - 1)[:, :, 0, 0], rtol=1e-12, atol=1e-12)
+    # middle branch
+    m = o.ladder_contract(a, b, "phys", "phys", start=1, end=3)
+    np.testing.assert_allclose(_labelled(m, ["left1", "left2", "right1", "right2"]),
+                               _transfer_chain(a, b, 1, 3), rtol=1e-12, atol=1e-12)
+    with pytest.raises(ValueError):
+        o.ladder_contract(a, b, "phys", "phys", start=3, end=2)
+
+
+def _xxz_target(N, chi, rng):
+    m = random_omps(N, 2, chi, rng)
+    return o.contract_mps_mpo(m, om.xxz_mpo(N, 1.0, 0.7, 0.3))
+
+
+def test_variational_compress_exact_when_chi_suffices(rng):
+    N = 6
+    tgt = _xxz_target(N, 3, rng)
+    v = om.mps_to_dense(tgt)
+    out = tgt.variational_compress(8, max_iter=20, tolerance=1e-8)  # 2^3 = exact bond bound
+    assert out.phys_label == tgt.phys_label and out.left_label == tgt.left_label
+    np.testing.assert_allclose(om.mps_to_dense(out), v, atol=1e-10 * np.linalg.norm(v))
+    assert out.check_canonical_form(threshold=1e-10)[0] == N - 1
+
+
+@pytest.mark.parametrize("chi", [2, 4])
+def test_variational_compress_projection_identities(chi, rng):
+    N = 9
+    tgt = _xxz_target(N, 4, rng)
+    v = om.mps_to_dense(tgt)
+    s = tgt.copy(); s.svd_compress(chi=chi)
+    out = o.variational_compress_mps(tgt, chi, max_iter=60, tolerance=1e-8)
+    w = om.mps_to_dense(out)
+    d_var = np.linalg.norm(v - w)
+    # never worse than its svd_compress starting point; bonds stay within chi
+    assert d_var <= np.linalg.norm(v - om.mps_to_dense(s)) * (1 + 1e-12)
+    assert max(out.bonddims()) <= chi
+    # left canonical; the last site is the orthogonal projection of the target
+    # onto the span fixed by the other sites: site == (left isometry)^T target,
+    # hence <w, v> == |w|^2 and |v - w|^2 == |v|^2 - |w|^2
+    assert out.check_canonical_form(threshold=1e-10)[0] == N - 1
+    np.testing.assert_allclose(w @ v, w @ w, rtol=1e-9)
+    np.testing.assert_allclose(d_var ** 2, v @ v - w @ w, rtol=1e-7, atol=1e-12 * (v @ v))
+    # the functional variant and the method agree
+    out2 = tgt.variational_compress(chi, max_iter=60, tolerance=1e-8)
+    np.testing.assert_allclose(om.mps_to_dense(out2), w, atol=1e-12 * np.linalg.norm(w))
+
+
+def test_variational_compress_target_unchanged_and_nonconvergence(rng):
+    tgt = _xxz_target(7, 3, rng)
+    v = om.mps_to_dense(tgt)
+    tgt.variational_compress(3, max_iter=10, tolerance=1e-6)
+    np.testing.assert_allclose(om.mps_to_dense(tgt), v, atol=1e-12 * np.linalg.norm(v))
+    with pytest.raises(RuntimeError):
+        tgt.variational_compress(2, max_iter=1, tolerance=1e-30)
