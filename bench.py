@@ -61,10 +61,11 @@ def parse():
     p.add_argument("--traffic", type=float, default=None,
                    help="HBM bytes per launch of the dominant kernel, from the calibrated "
                         "FETCH_SIZE + WRITE_SIZE counters (reported as roofline.traffic)")
-    p.add_argument("--workload", default="chain", choices=["chain", "batched"],
+    p.add_argument("--workload", default="chain", choices=["chain", "batched", "variational"],
                    help="chain: the MPO.MPS compress sweep (default, the headline); batched: "
                         "1024 uneven contraction + truncation jobs (chi ~ U{16..512}, d=2, "
-                        "FP32 data, keep sigma/sigma_0 > 1e-8)")
+                        "FP32 data, keep sigma/sigma_0 > 1e-8); variational: iterations/s of "
+                        "variational_compress on the MPO-applied chain")
     p.add_argument("--jobs", type=int, default=1024)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                    help="only run a plain device copy of MB megabytes three times (the "
@@ -211,6 +212,134 @@ def batched_workload(args):
     print(json.dumps(rec), flush=True)
 
 
+def roofline_from_prof(prof, traffic=None):
+    """Roofline record for the tag with the largest measured time."""
+    dom = max(prof, key=lambda t: prof[t][0])
+    ms, n, fl, by = prof[dom]
+    if dom in PEAK_TFLOPS and fl > 0:
+        ach = fl / (ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_TFLOPS[dom],
+                "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dom],
+                "traffic": traffic if traffic is not None else MEASURED_TRAFFIC.get(dom),
+                "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n,
+                "flops_per_launch": fl / max(n, 1)}
+    else:
+        ach = by / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": 8000.0, "unit": "GB/s",
+                "frac": ach / 8000.0,
+                "traffic": traffic if traffic is not None else MEASURED_TRAFFIC.get(dom),
+                "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n,
+                "bytes_per_launch": by / max(n, 1)}
+    if dom in NOTES:
+        roof["note"] = NOTES[dom]
+    return roof
+
+
+def variational_cpu_sample(chi, chiD, d=2):
+    """The oracle's bulk site update of variational_compress (onedim_core.py:
+    548-597) at the workload's shapes: target . left env, . right env, LQ,
+    L into the neighbour, right-environment update."""
+    import numpy as np
+    from oracle import tn_oracle as o
+    rng = np.random.default_rng(2)
+    tgt = o.OTensor(rng.standard_normal((chiD, d, chiD)), ["left", "phys", "right"])
+    le = o.OTensor(rng.standard_normal((chi, chiD)), ["le1", "le2"])
+    re = o.OTensor(rng.standard_normal((chi, chiD)), ["re1", "re2"])
+    nb = o.OTensor(rng.standard_normal((chi, d, chi)), ["l", "p", "r"])
+    t0 = time.perf_counter()
+    upd = o.contract(tgt, le, "left", "le2")
+    upd = o.contract(upd, re, "right", "re2")
+    Lt, Q = o.tensor_lq(upd, "le1", lq_label="lq")
+    o.contract(nb, Lt, "r", "le1")
+    r2 = o.contract(Q, re, "re1", "re1")
+    o.contract(r2, tgt, ["phys", "re2"], ["phys", "right"])
+    return time.perf_counter() - t0
+
+
+def variational_workload(args):
+    """Iterations/s of MatrixProductState.variational_compress on the XXZ MPO
+    applied to the workload MPS (target bond chi*D), compressed to chi, with
+    the structured sweep's output as the initial guess.  One step = one full
+    iteration of the reference loop (a right-to-left and a mirrored sweep)."""
+    import numpy as np
+    import torch
+    import tncontract_amd as tn
+    from tncontract_amd import _lib as L
+    from tncontract_amd.onedim import random_mps_arrays, xxz_mpo
+    from tncontract_amd.onedim.onedim_core import _variational_compress
+    from tncontract_amd.onedim.structured import apply_mpo_compress
+    torch.cuda.set_device(0)
+    N, d, chi, D = args.N, 2, args.chi, 5
+    arrs = random_mps_arrays(N, d, chi, seed=args.seed)
+    psi = tn.onedim.MatrixProductState([tn.Tensor(a, ["phys", "left", "right"]) for a in arrs])
+    mpo = xxz_mpo(N)
+    target = tn.onedim.contract_mps_mpo(psi, mpo)
+    guess = apply_mpo_compress(psi, mpo, chi=chi, threshold=1e-15, precision="fp64")
+    torch.cuda.synchronize()
+    log("variational: target bonds %d, guess bonds %d" % (max(target.bonddims()), max(guess.bonddims())))
+
+    def run(iters, prof=False):
+        g = guess.copy()
+        torch.cuda.synchronize()
+        if prof:
+            L.prof_collect("gemm_f64", reset=True)
+            L.prof_enable(True)
+        t0 = time.perf_counter()
+        out, it, _ = _variational_compress(target, chi, iters, g, 0.0, False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if prof:
+            L.prof_enable(False)
+        assert it == iters
+        return out, dt
+
+    log("variational warmup")
+    run(1)
+    # one-iteration run gives the fixed set-up cost (QR left sweep of the
+    # guess + left environments) plus one iteration; the timed run adds steps
+    _, t1 = run(1)
+    log("variational timed x%d" % args.steps)
+    out, tk = run(1 + args.steps)
+    dt = (tk - t1) / args.steps
+    # projection identities of the final single-site update (exact at any
+    # iteration count): <w|v> = <w|w>, |v - w|^2 = |v|^2 - |w|^2
+    ip = tn.onedim.inner_product_mps
+    vv, ww, wv = float(ip(target, target)), float(ip(out, out)), float(ip(out, target))
+    proj_err = abs(wv - ww) / ww
+    dist_var = (vv - ww) / vv
+    dist_guess = float(tn.onedim.frob_distance_squared(guess, target)) / vv
+    L.prof_collect("gemm_f64", reset=True)
+    _, _ = run(1, prof=True)
+    prof = {tag: L.prof_collect(tag) for tag in L.PROF_TAGS}
+    L.prof_collect("gemm_f64", reset=True)
+    roof = roofline_from_prof(prof, args.traffic)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline sample")
+        ts = variational_cpu_sample(chi, chi * D, d)
+        # an iteration visits 2(N-1) site updates; edge sites are smaller,
+        # counted as bulk (the sample overstates the CPU rate slightly)
+        per_iter = ts * 2 * (N - 1)
+        cpu = {"value": 1.0 / per_iter, "unit": "iterations/s", "cores": len(os.sched_getaffinity(0)),
+               "kind": "port", "sample": "NumPy FP64 oracle, one bulk site update of "
+               "variational_compress at chi=%d, target bond %d (%.2f s), times the %d site "
+               "updates of an iteration" % (chi, chi * D, ts, 2 * (N - 1))}
+    rec = {"metric": "variational_compress iterations/s", "value": 1.0 / dt,
+           "unit": "iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded random MPS, i.i.d. N(0,1), normalised; XXZ MPO)",
+           "config": {"workload": "variational_compress of XXZ MPO D=5 x random MPS N=%d d=%d "
+                                  "chi=%d (target bond %d) to chi=%d, structured-sweep initial guess"
+                                  % (N, d, chi, chi * D, chi)},
+           "setup_plus_one_iteration_s": t1,
+           "rel_dist2_guess": dist_guess, "rel_dist2_variational": dist_var,
+           "projection_identity_rel_err": proj_err,
+           "kernel_ms_profiled_iteration": {t: round(v[0], 3) for t, v in prof.items() if v[1]},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -222,6 +351,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         batched_workload(args)
+        return
+    if args.workload == "variational":
+        variational_workload(args)
         return
     import numpy as np
     import torch
@@ -268,24 +400,7 @@ def main():
     L.prof_enable(False)
     prof = {tag: L.prof_collect(tag) for tag in L.PROF_TAGS}
     L.prof_collect("gemm_f64", reset=True)
-    dom = max(prof, key=lambda t: prof[t][0])
-    ms, n, fl, by = prof[dom]
-    if dom in PEAK_TFLOPS and fl > 0:
-        ach = fl / (ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_TFLOPS[dom],
-                "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dom],
-                "traffic": args.traffic if args.traffic is not None else MEASURED_TRAFFIC.get(dom),
-                "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n,
-                "flops_per_launch": fl / max(n, 1)}
-    else:
-        ach = by / (ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": 8000.0, "unit": "GB/s",
-                "frac": ach / 8000.0,
-                "traffic": args.traffic if args.traffic is not None else MEASURED_TRAFFIC.get(dom),
-                "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n,
-                "bytes_per_launch": by / max(n, 1)}
-    if dom in NOTES:
-        roof["note"] = NOTES[dom]
+    roof = roofline_from_prof(prof, args.traffic)
     kernel_ms = {t: round(v[0], 3) for t, v in prof.items() if v[1]}
 
     # ---- plain FP64 path (the denominator) and the error between the two

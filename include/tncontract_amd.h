@@ -101,6 +101,20 @@ int tn_contract(const void* A, const tn_modes* a_m, const tn_modes* a_k,
                 void* C, const tn_modes* c_m, const tn_modes* c_n,
                 int dtype, int precision, double alpha, double beta, void* stream);
 
+/* tn_contract with a split-K workspace (same contraction; no counterpart in
+ * the reference, whose tensordot has no device scratch).  When the output has
+ * fewer 128 x 128 tiles than two per CU, K is split over workgroups whose
+ * partial sums go to ws and are reduced into C by a second launch.
+ * tn_contract_workspace gives the bytes for the full split of an M x N x K
+ * contraction (0: no split); a smaller ws_bytes only limits the split, and
+ * ws == NULL runs tn_contract.  Calls sharing one ws must be stream-ordered. */
+size_t tn_contract_workspace(int64_t M, int64_t N, int64_t K, int dtype);
+int tn_contract_ws(const void* A, const tn_modes* a_m, const tn_modes* a_k,
+                   const void* B, const tn_modes* b_k, const tn_modes* b_n,
+                   void* C, const tn_modes* c_m, const tn_modes* c_n,
+                   int dtype, int precision, double alpha, double beta,
+                   void* ws, size_t ws_bytes, void* stream);
+
 /* ---- QR (np.linalg.qr(mode="reduced") in tensor_qr, tensor.py:1044) ------
  * A is the m x n matrix whose row index is decoded through a_rows and column
  * index through a_cols.  Q: m x k row-major, R: k x n row-major, k=min(m,n).

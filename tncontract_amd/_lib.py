@@ -80,6 +80,8 @@ def lib():
         "tn_normalize_workspace": (sz, [i64]),
         "tn_normalize": (i32, [vp, i32, i64, vp, vp, sz, vp]),
         "tn_contract": (i32, [vp, mp, mp, vp, mp, mp, vp, mp, mp, i32, i32, dbl, dbl, vp]),
+        "tn_contract_workspace": (sz, [i64, i64, i64, i32]),
+        "tn_contract_ws": (i32, [vp, mp, mp, vp, mp, mp, vp, mp, mp, i32, i32, dbl, dbl, vp, sz, vp]),
         "tn_qr_workspace": (sz, [i64, i64, i32]),
         "tn_qr": (i32, [vp, mp, mp, i64, i64, vp, vp, i32, vp, sz, vp]),
         "tn_prof_enable": (i32, [i32]),
@@ -165,6 +167,28 @@ def dtype_code(t):
 
 def workspace(nbytes):
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
+
+
+_SPLITK_WS = {}
+
+
+def contract(A, am, ak, B, bk, bn, C, cm, cn, dtype_code, prec, M, N, K, alpha=1.0, beta=0.0):
+    """tn_contract through the split-K variant: the partial-sum workspace is
+    one buffer per device, grown on demand and reused by every call (all
+    calls are ordered on the current stream)."""
+    need = lib().tn_contract_workspace(int(M), int(N), int(K), dtype_code)
+    ws, nb = None, 0
+    if need:
+        dev = torch.cuda.current_device()
+        ws = _SPLITK_WS.get(dev)
+        if ws is None or ws.numel() < need:
+            ws = torch.empty(max(int(need), 1 << 24), dtype=torch.uint8, device=device())
+            _SPLITK_WS[dev] = ws
+        nb = ws.numel()
+    check(lib().tn_contract_ws(ptr(A), ctypes.byref(am), ctypes.byref(ak), ptr(B), ctypes.byref(bk),
+                               ctypes.byref(bn), ptr(C), ctypes.byref(cm), ctypes.byref(cn),
+                               dtype_code, prec, alpha, beta, ptr(ws) if ws is not None else None,
+                               nb, stream_ptr()))
 
 
 def empty(shape, dtype=torch.float64):

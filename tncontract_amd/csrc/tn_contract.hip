@@ -455,6 +455,15 @@ __global__ void __launch_bounds__(256) splitk_reduce(const T* __restrict__ part,
   *p = (T)v;
 }
 
+// Split-K factor: with fewer output tiles than two per CU, split K so that
+// about 512 workgroups (two resident per CU) run, keeping >= 256 of K each.
+int splitk_factor(int64_t M, int64_t N, int64_t K) {
+  const int64_t tiles = ceil_div(M, BM) * ceil_div(N, BN);
+  if (tiles >= 256 || K < 512) return 1;
+  const int64_t want = (512 + tiles / 2) / tiles;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(want, K / 256), 64));
+}
+
 int contract_launch(const void* A, const Modes& a_m, const Modes& a_k, const void* B,
                     const Modes& b_k, const Modes& b_n, void* C, const Modes& c_m,
                     const Modes& c_n, int dtype, int prec, double alpha, double beta,
@@ -536,8 +545,8 @@ int contract_launch(const void* A, const Modes& a_m, const Modes& a_k, const voi
   // Split K when few output tiles would leave most CUs idle.
   int S = 1;
   const size_t esz = dtype == TN_F64 ? 8 : 4;
-  if (splitk_ws != nullptr && tiles < 128 && K >= 1024) {
-    S = (int)std::min<int64_t>(std::min<int64_t>(ceil_div(256, tiles), K / 256), 64);
+  if (splitk_ws != nullptr) {
+    S = splitk_factor(M, N, K);
     while (S > 1 && (size_t)S * M * N * esz > splitk_bytes) --S;
   }
   g.kchunk = S > 1 ? ceil_div(ceil_div(K, S), 32) * 32 : K;
@@ -603,4 +612,28 @@ extern "C" int tn_contract(const void* A, const tn_modes* a_m, const tn_modes* a
     for (int i = 0; i < m->n; ++i) TN_REQUIRE(m->dims[i] >= 0, "tn_contract: negative dimension");
   return contract_launch(A, am, ak, B, bk, bn, C, cm, cn, dtype, precision, alpha, beta, nullptr,
                          0, as_stream(stream));
+}
+
+extern "C" size_t tn_contract_workspace(int64_t M, int64_t N, int64_t K, int dtype) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  const int S = splitk_factor(M, N, K);
+  return S > 1 ? (size_t)S * (size_t)M * (size_t)N * (dtype == TN_F64 ? 8 : 4) : 0;
+}
+
+extern "C" int tn_contract_ws(const void* A, const tn_modes* a_m, const tn_modes* a_k,
+                              const void* B, const tn_modes* b_k, const tn_modes* b_n, void* C,
+                              const tn_modes* c_m, const tn_modes* c_n, int dtype, int precision,
+                              double alpha, double beta, void* ws, size_t ws_bytes, void* stream) {
+  for (const tn_modes* m : {a_m, a_k, b_k, b_n, c_m, c_n})
+    TN_REQUIRE(m == nullptr || (m->nmodes >= 0 && m->nmodes <= TN_MAX_MODES),
+               "tn_contract_ws: mode group has too many modes");
+  TN_REQUIRE(dims_equal(a_m, c_m), "tn_contract_ws: A row modes and C row modes differ");
+  TN_REQUIRE(dims_equal(b_n, c_n), "tn_contract_ws: B col modes and C col modes differ");
+  TN_REQUIRE(dims_equal(a_k, b_k), "tn_contract_ws: contracted modes of A and B differ");
+  Modes am = to_modes(a_m), ak = to_modes(a_k), bk = to_modes(b_k), bn = to_modes(b_n),
+        cm = to_modes(c_m), cn = to_modes(c_n);
+  for (const Modes* m : {&am, &ak, &bn})
+    for (int i = 0; i < m->n; ++i) TN_REQUIRE(m->dims[i] >= 0, "tn_contract_ws: negative dimension");
+  return contract_launch(A, am, ak, B, bk, bn, C, cm, cn, dtype, precision, alpha, beta,
+                         ws_bytes ? ws : nullptr, ws ? ws_bytes : 0, as_stream(stream));
 }

@@ -244,71 +244,11 @@ class MatrixProductState(OneDimensionalTensorNetwork):
         relative changes of the site norms along the second sweep are all
         below `tolerance` (RuntimeError after `max_iter` full updates).
         Returns a new, left-canonical MPS with this MPS's labels."""
-        if initial_guess is None:
-            mps = self.copy()
-            mps.svd_compress(chi=chi, reverse=True)
-        else:
-            mps = initial_guess
-            mps.left_canonise(qr_decomposition=True)
-        mps.replace_labels([mps.left_label, mps.right_label, mps.phys_label],
-                           [unique_label(), unique_label(), unique_label()])
-        le_label = unique_label()
-        left_env = ladder_contract(mps, self, mps.phys_label, self.phys_label,
-                                   return_intermediate_contractions=True,
-                                   right_output_label=le_label, complex_conjugate_array1=True)
-        target = self
-
-        def sweep(mps1, mps2, left_env):
-            le = left_env[0].labels[0][:-1]
-            re = unique_label()
-            lq = unique_label()
-            right_envs = []
-            norms = [mps1[-1].norm()]
-            renv = None
-            for i in range(mps2.nsites - 1, 0, -1):
-                upd = tsr.contract(mps2[i], left_env[i - 1], mps2.left_label, le + "2")
-                if i != mps2.nsites - 1:
-                    upd = tsr.contract(upd, renv, mps2.right_label, re + "2")
-                    upd.replace_label(re + "1", mps1.right_label)
-                upd.replace_label([le + "1", mps2.phys_label], [mps1.left_label, mps1.phys_label])
-                Lt, Q = tsr.tensor_lq(upd, mps1.left_label, lq_label=lq)
-                Q.replace_label(lq + "out", mps1.left_label)
-                Lt.replace_label(lq + "in", mps1.right_label)
-                mps1[i] = Q
-                mps1[i - 1] = tsr.contract(mps1[i - 1], Lt, mps1.right_label, mps1.left_label)
-                norms.append(mps1[i - 1].norm())
-                if i == mps2.nsites - 1:
-                    renv = tsr.contract(tsr.conjugate(mps1[i]), mps2[i], mps1.phys_label,
-                                        target.phys_label)
-                    renv.remove_all_dummy_indices(labels=[mps1.right_label, mps2.right_label])
-                else:
-                    renv.contract(tsr.conjugate(mps1[i]), re + "1", mps1.right_label)
-                    renv.contract(mps2[i], [mps1.phys_label, re + "2"],
-                                  [target.phys_label, target.right_label])
-                renv.replace_label([mps1.left_label, mps2.left_label], [re + "1", re + "2"])
-                right_envs.append(renv.copy())
-                if i == 1:
-                    upd = tsr.contract(mps2[0], renv, mps2.right_label, re + "2")
-                    upd.replace_label([mps2.phys_label, re + "1"],
-                                      [mps1.phys_label, mps1.right_label])
-                    mps1[0] = upd
-            return right_envs, np.array(norms)
-
-        for it in range(max_iter):
-            left_env, norms1 = sweep(mps, self, left_env)
-            mps.reverse()
-            self.reverse()
-            left_env, norms2 = sweep(mps, self, left_env)
-            mps.reverse()
-            self.reverse()
-            if np.all(np.abs(norms2[1:] - norms2[:-1]) / norms2[1:] < tolerance):
-                mps.replace_labels([mps.left_label, mps.right_label, mps.phys_label],
-                                   [self.left_label, self.right_label, self.phys_label])
-                if normalise:
-                    mps[-1].scale_(1.0 / mps.norm(canonical_form="left"))
-                return mps
-            elif it == max_iter - 1:
-                raise RuntimeError("variational_compress did not converge.")
+        mps, _, converged = _variational_compress(self, chi, max_iter, initial_guess, tolerance,
+                                                  normalise)
+        if not converged:
+            raise RuntimeError("variational_compress did not converge.")
+        return mps
 
     def check_canonical_form(self, threshold=1e-14, print_output=True):
         """onedim_core.py:408-461"""
@@ -358,6 +298,78 @@ class MatrixProductOperator(OneDimensionalTensorNetwork):
 
     def physindim(self, site):
         return self.data[site].index_dimension(self.physin_label)
+
+
+def _variational_compress(self, chi, max_iter, initial_guess, tolerance, normalise):
+    """Body of MatrixProductState.variational_compress: returns the compressed
+    MPS (relabelled, normalised on request), the number of full iterations
+    run and whether the norm criterion was met."""
+    if initial_guess is None:
+        mps = self.copy()
+        mps.svd_compress(chi=chi, reverse=True)
+    else:
+        mps = initial_guess
+        mps.left_canonise(qr_decomposition=True)
+    mps.replace_labels([mps.left_label, mps.right_label, mps.phys_label],
+                       [unique_label(), unique_label(), unique_label()])
+    le_label = unique_label()
+    left_env = ladder_contract(mps, self, mps.phys_label, self.phys_label,
+                               return_intermediate_contractions=True,
+                               right_output_label=le_label, complex_conjugate_array1=True)
+    target = self
+
+    def sweep(mps1, mps2, left_env):
+        le = left_env[0].labels[0][:-1]
+        re = unique_label()
+        lq = unique_label()
+        right_envs = []
+        norms = [mps1[-1].norm()]
+        renv = None
+        for i in range(mps2.nsites - 1, 0, -1):
+            upd = tsr.contract(mps2[i], left_env[i - 1], mps2.left_label, le + "2")
+            if i != mps2.nsites - 1:
+                upd = tsr.contract(upd, renv, mps2.right_label, re + "2")
+                upd.replace_label(re + "1", mps1.right_label)
+            upd.replace_label([le + "1", mps2.phys_label], [mps1.left_label, mps1.phys_label])
+            Lt, Q = tsr.tensor_lq(upd, mps1.left_label, lq_label=lq)
+            Q.replace_label(lq + "out", mps1.left_label)
+            Lt.replace_label(lq + "in", mps1.right_label)
+            mps1[i] = Q
+            mps1[i - 1] = tsr.contract(mps1[i - 1], Lt, mps1.right_label, mps1.left_label)
+            norms.append(mps1[i - 1].norm())
+            if i == mps2.nsites - 1:
+                renv = tsr.contract(tsr.conjugate(mps1[i]), mps2[i], mps1.phys_label,
+                                    target.phys_label)
+                renv.remove_all_dummy_indices(labels=[mps1.right_label, mps2.right_label])
+            else:
+                renv.contract(tsr.conjugate(mps1[i]), re + "1", mps1.right_label)
+                renv.contract(mps2[i], [mps1.phys_label, re + "2"],
+                              [target.phys_label, target.right_label])
+            renv.replace_label([mps1.left_label, mps2.left_label], [re + "1", re + "2"])
+            right_envs.append(renv.copy())
+            if i == 1:
+                upd = tsr.contract(mps2[0], renv, mps2.right_label, re + "2")
+                upd.replace_label([mps2.phys_label, re + "1"],
+                                  [mps1.phys_label, mps1.right_label])
+                mps1[0] = upd
+        return right_envs, np.array(norms)
+
+    converged, it = False, -1
+    for it in range(max_iter):
+        left_env, norms1 = sweep(mps, self, left_env)
+        mps.reverse()
+        self.reverse()
+        left_env, norms2 = sweep(mps, self, left_env)
+        mps.reverse()
+        self.reverse()
+        if np.all(np.abs(norms2[1:] - norms2[:-1]) / norms2[1:] < tolerance):
+            converged = True
+            break
+    mps.replace_labels([mps.left_label, mps.right_label, mps.phys_label],
+                       [self.left_label, self.right_label, self.phys_label])
+    if normalise:
+        mps[-1].scale_(1.0 / mps.norm(canonical_form="left"))
+    return mps, it + 1, converged
 
 
 def contract_virtual_indices(array_1d, start=0, end=None, periodic_boundaries=True):
@@ -551,9 +563,9 @@ def _contract_consolidated(t1, t2, l1, l2):
     cn = L.modes(dims[nf1:], stride[nf1:])
     if len(ax1) != len(ax2) or any(t1.shape[a] != t2.shape[b] for a, b in zip(ax1, ax2)):
         raise ValueError("contract_mps_mpo: physical dimensions do not match")
-    L.check(L.lib().tn_contract(L.ptr(d1), ctypes.byref(am), ctypes.byref(ak), L.ptr(d2),
-                                ctypes.byref(bk), ctypes.byref(bn), L.ptr(out), ctypes.byref(cm),
-                                ctypes.byref(cn), L.dtype_code(d1), 0, 1.0, 0.0, L.stream_ptr()))
+    L.contract(d1, am, ak, d2, bk, bn, out, cm, cn, L.dtype_code(d1), 0,
+               int(np.prod([d1.shape[a] for a in free1])), int(np.prod([d2.shape[a] for a in free2])),
+               int(np.prod([d1.shape[a] for a in ax1])))
     shape = [int(np.prod([dims[a] for a in g])) for g in groups]
     return tsr.Tensor._wrap(out.view(shape), labs)
 

@@ -413,9 +413,10 @@ def _contract_data(d1, ax1, d2, ax2, precision="native"):
     bn = L.tensor_modes(d2, free2)
     cm = L.modes(out_shape[:nf1], strides[:nf1])
     cn = L.modes(out_shape[nf1:], strides[nf1:])
-    L.check(L.lib().tn_contract(L.ptr(d1), ctypes.byref(am), ctypes.byref(ak), L.ptr(d2),
-                                ctypes.byref(bk), ctypes.byref(bn), L.ptr(out), ctypes.byref(cm),
-                                ctypes.byref(cn), L.dtype_code(d1), prec, 1.0, 0.0, L.stream_ptr()))
+    M = int(np.prod(out_shape[:nf1])) if nf1 else 1
+    N = int(np.prod(out_shape[nf1:])) if len(out_shape) > nf1 else 1
+    K = int(np.prod([d1.shape[a] for a in ax1])) if ax1 else 1
+    L.contract(d1, am, ak, d2, bk, bn, out, cm, cn, L.dtype_code(d1), prec, M, N, K)
     return out
 
 
