@@ -834,6 +834,34 @@ def frob_distance_squared(mps1, mps2):
     return ip(mps1, mps1) + ip(mps2, mps2) - 2 * np.real(ip(mps1, mps2))
 
 
+def contract_virtual_indices(array_1d, start=0, end=None, periodic_boundaries=True):
+    """onedim_core.py:1399-1422 -- contract the virtual bonds of
+    array_1d[start:end] (the slice excludes `end`), then, with periodic
+    boundaries, trace the outer left/right indices."""
+    C = array_1d[start].copy()
+    for x in array_1d[start + 1:end]:
+        C = contract(C, x, array_1d.right_label, array_1d.left_label)
+    if periodic_boundaries:
+        C.contract_internal(array_1d.right_label, array_1d.left_label)
+    return C
+
+
+def contract_multi_index_tensor_with_one_dim_array(tensor, array, label1, label2):
+    """onedim_core.py:1370-1396 -- the label2 index of array[i] is contracted
+    with the first still-open label1 index of `tensor` (the label1 indices are
+    renamed to a temporary label for the duration), the virtual bonds along
+    the way, and finally the array's outer left/right indices."""
+    tmp = unique_label()
+    tensor.replace_label(label1, tmp)
+    C = contract(tensor, array[0], tmp, label2, index_slice1=[0])
+    for i in range(1, len(array)):
+        C = contract(C, array[i], [array.right_label, tmp], [array.left_label, label2],
+                     index_slice1=[0, 1])
+    C.contract_internal(array.right_label, array.left_label)
+    tensor.replace_label(tmp, label1)
+    return C
+
+
 def left_canonical_form_mps(orig_mps, chi=0, threshold=1e-14, normalise=False):
     """onedim_core.py:1425-1436"""
     m = orig_mps.copy()

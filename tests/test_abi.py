@@ -47,3 +47,28 @@ def test_error_path_without_gpu_calls():
     assert b"modes" in lib.tn_last_error()
     with pytest.raises(ValueError):
         L.check(rc)
+
+
+def test_merged_modes_reproduce_view_offsets():
+    """The fused mode group of a strided view enumerates, in row-major
+    order, exactly the element offsets of the view (host logic of the
+    permute launches; no GPU call)."""
+    import itertools
+    import numpy as np
+    from tncontract_amd._lib import _merged_modes
+    rng = np.random.default_rng(0)
+    for trial in range(200):
+        nd = int(rng.integers(1, 11))
+        shape = [int(x) for x in rng.integers(1, 4, nd)]
+        base = np.arange(int(np.prod(shape))).reshape(shape)
+        perm = list(rng.permutation(nd))
+        view = base.transpose(perm)
+        if trial % 3 == 0 and view.ndim and view.shape[0] > 1:
+            view = view[1:]
+        strides = [s // base.itemsize for s in view.strides]
+        dims, st = _merged_modes(view.shape, strides)
+        assert all(d > 1 for d in dims)
+        off0 = view.reshape(-1)[0] if view.size else 0
+        offs = [off0 + sum(i * s for i, s in zip(idx, st))
+                for idx in itertools.product(*[range(d) for d in dims])]
+        assert offs == list(view.reshape(-1))

@@ -12,6 +12,7 @@ from tncontract_amd.tensor import tensor_qr, tensor_lq, conjugate  # noqa: F401
 from tncontract_amd.label import *  # noqa: F401,F403
 from tncontract_amd.tncon import con  # noqa: F401
 from tncontract_amd import onedim  # noqa: F401
+from tncontract_amd import twodim  # noqa: F401
 from tncontract_amd import _lib  # noqa: F401
 
 

@@ -200,14 +200,45 @@ def tensor_modes(t, axes):
     return modes([t.shape[a] for a in axes], [t.stride(a) for a in axes])
 
 
+def _merged_modes(shape, strides):
+    """Axes of a strided view with size-1 axes dropped and neighbours whose
+    strides nest (stride[i] == stride[i+1] * dim[i+1]) fused; the dense
+    row-major copy of the view has the same fused axes."""
+    dims, st = [], []
+    for d, s in zip(shape, strides):
+        if d == 1:
+            continue
+        if dims and st[-1] == s * d:
+            dims[-1] *= d
+            st[-1] = s
+        else:
+            dims.append(int(d))
+            st.append(int(s))
+    return dims, st
+
+
+def _permute_into(src, dst):
+    """dst (dense, same shape) <- src (strided view) through tn_permute; a view
+    with more than TN_MAX_MODES unfusable axes is copied slice by slice along
+    its outermost non-trivial axis."""
+    dims, st = _merged_modes(src.shape, src.stride())
+    if len(dims) <= TN_MAX_MODES:
+        m = modes(dims, st) if dims else modes([1], [0])
+        check(lib().tn_permute(ctypes.c_void_p(src.data_ptr()), ctypes.byref(m), ptr(dst),
+                               dtype_code(src), stream_ptr()))
+        return
+    a = next(i for i, d in enumerate(src.shape) if d > 1)
+    for i in range(src.shape[a]):
+        _permute_into(src.select(a, i), dst.select(a, i))
+
+
 def materialize(t):
-    """Dense row-major copy of a strided device tensor through tn_permute."""
+    """Dense row-major copy of a strided device tensor through tn_permute
+    (the storage offset is folded into data_ptr)."""
     out = torch.empty(tuple(t.shape), dtype=t.dtype, device=t.device)
     if t.numel() == 0:
         return out
-    m = tensor_modes(t, range(t.dim())) if t.dim() > 0 else modes([1], [0])
-    src = t.data_ptr()  # storage offset already folded into data_ptr
-    check(lib().tn_permute(ctypes.c_void_p(src), ctypes.byref(m), ptr(out), dtype_code(t), stream_ptr()))
+    _permute_into(t, out)
     return out
 
 

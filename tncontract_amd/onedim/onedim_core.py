@@ -20,7 +20,8 @@ __all__ = ["MatrixProductState", "MatrixProductOperator", "OneDimensionalTensorN
            "check_canonical_form_mps", "contract_mps_mpo", "frob_distance_squared",
            "inner_product_mps", "ladder_contract", "left_canonical_form_mps",
            "mps_complex_conjugate", "reverse_mps", "right_canonical_form_mps",
-           "svd_compress_mps", "variational_compress_mps", "contract_virtual_indices"]
+           "svd_compress_mps", "variational_compress_mps", "contract_virtual_indices",
+           "contract_multi_index_tensor_with_one_dim_array"]
 
 
 class OneDimensionalTensorNetwork:
@@ -379,6 +380,22 @@ def contract_virtual_indices(array_1d, start=0, end=None, periodic_boundaries=Tr
         C = tsr.contract(C, x, array_1d.right_label, array_1d.left_label)
     if periodic_boundaries:
         C.contract_internal(array_1d.right_label, array_1d.left_label)
+    return C
+
+
+def contract_multi_index_tensor_with_one_dim_array(tensor, array, label1, label2):
+    """onedim_core.py:1370-1396: the label2 index of array[i] is contracted
+    with the first still-open label1 index of `tensor` (temporarily renamed),
+    together with the array's virtual bonds; finally the array's outer
+    left/right indices are traced.  Each step is one MFMA contraction."""
+    tmp = unique_label()
+    tensor.replace_label(label1, tmp)
+    C = tsr.contract(tensor, array[0], tmp, label2, index_slice1=[0])
+    for i in range(1, len(array)):
+        C = tsr.contract(C, array[i], [array.right_label, tmp], [array.left_label, label2],
+                         index_slice1=[0, 1])
+    C.contract_internal(array.right_label, array.left_label)
+    tensor.replace_label(tmp, label1)
     return C
 
 
