@@ -61,11 +61,15 @@ def parse():
     p.add_argument("--traffic", type=float, default=None,
                    help="HBM bytes per launch of the dominant kernel, from the calibrated "
                         "FETCH_SIZE + WRITE_SIZE counters (reported as roofline.traffic)")
-    p.add_argument("--workload", default="chain", choices=["chain", "batched", "variational"],
+    p.add_argument("--workload", default="chain", choices=["chain", "batched", "variational", "peps"],
                    help="chain: the MPO.MPS compress sweep (default, the headline); batched: "
                         "1024 uneven contraction + truncation jobs (chi ~ U{16..512}, d=2, "
                         "FP32 data, keep sigma/sigma_0 > 1e-8); variational: iterations/s of "
-                        "variational_compress on the MPO-applied chain")
+                        "variational_compress on the MPO-applied chain; peps: boundary-MPS "
+                        "inner_product_peps contractions/s")
+    p.add_argument("--peps-L", type=int, default=10)
+    p.add_argument("--peps-D", type=int, default=3)
+    p.add_argument("--peps-chi", type=int, default=64)
     p.add_argument("--jobs", type=int, default=1024)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                    help="only run a plain device copy of MB megabytes three times (the "
@@ -340,6 +344,88 @@ def variational_workload(args):
     print(json.dumps(rec), flush=True)
 
 
+def peps_arrays(L, D, d, seed):
+    """Seeded random L x L PEPS site arrays (phys, up, right, down, left),
+    i.i.d. N(0,1) scaled by 1/sqrt(d D^2) so the network stays O(1)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(L):
+        row = []
+        for j in range(L):
+            shape = (d, 1 if i == 0 else D, 1 if j == L - 1 else D, 1 if i == L - 1 else D,
+                     1 if j == 0 else D)
+            row.append(rng.standard_normal(shape) / np.sqrt(d * D * D))
+        out.append(row)
+    return out
+
+
+def peps_workload(args):
+    """Contractions/s of inner_product_peps(psi, psi) by boundary MPS
+    (square_lattice.py:136-203, 281-304): L x L PEPS of bond D, double-layer
+    bond D^2, boundary MPS compressed to chi by svd_compress_mps per column.
+    One step = one full contraction (all columns)."""
+    import numpy as np
+    import torch
+    import tncontract_amd as tn
+    from tncontract_amd import _lib as L
+    torch.cuda.set_device(0)
+    Ls, D, d, chi = args.peps_L, args.peps_D, 2, args.peps_chi
+    labels = ["phys", "up", "right", "down", "left"]
+    arrs = peps_arrays(Ls, D, d, args.seed)
+    psi = tn.twodim.SquareLatticePEPS([[tn.Tensor(a, list(labels)) for a in row] for row in arrs])
+    torch.cuda.synchronize()
+
+    def step():
+        return tn.twodim.inner_product_peps(psi, psi, exact_contract=False, chi=chi)
+
+    log("peps: L=%d D=%d chi=%d warmup" % (Ls, D, chi))
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log("peps timed x%d" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        val = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    value = float(val.numpy().reshape(-1)[0])
+    L.prof_collect("gemm_f64", reset=True)
+    L.prof_enable(True)
+    step()
+    torch.cuda.synchronize()
+    L.prof_enable(False)
+    prof = {tag: L.prof_collect(tag) for tag in L.PROF_TAGS}
+    L.prof_collect("gemm_f64", reset=True)
+    roof = roofline_from_prof(prof, args.traffic)
+    cpu, rel_err = None, None
+    if not args.no_cpu:
+        from oracle import tn_oracle as o
+        from oracle import square_lattice_oracle as sq
+        log("cpu baseline: oracle on the same network")
+        ops = sq.OPEPS([[o.OTensor(a, list(labels)) for a in row] for row in arrs])
+        t1 = time.perf_counter()
+        ref = sq.inner_product_peps(ops, ops, exact_contract=False, chi=chi)
+        tc = time.perf_counter() - t1
+        refv = float(np.asarray(ref.data, dtype=np.float64).reshape(-1)[0])
+        rel_err = abs(value - refv) / abs(refv)
+        cpu = {"value": 1.0 / tc, "unit": "contractions/s", "cores": len(os.sched_getaffinity(0)),
+               "kind": "port", "sample": "NumPy FP64 oracle, the same full contraction once "
+                                         "(%.2f s)" % tc}
+    rec = {"metric": "inner_product_peps boundary-MPS contractions/s", "value": 1.0 / dt,
+           "unit": "contractions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded random PEPS, i.i.d. N(0,1)/sqrt(d D^2))",
+           "config": {"workload": "inner_product_peps(psi, psi), %dx%d PEPS d=%d D=%d "
+                                  "(double-layer bond %d), mps_contract svd chi=%d"
+                                  % (Ls, Ls, d, D, D * D, chi)},
+           "value_norm2": value, "rel_err_vs_oracle": rel_err,
+           "kernel_ms_profiled_step": {t: round(v[0], 3) for t, v in prof.items() if v[1]},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -354,6 +440,9 @@ def main():
         return
     if args.workload == "variational":
         variational_workload(args)
+        return
+    if args.workload == "peps":
+        peps_workload(args)
         return
     import numpy as np
     import torch

@@ -220,3 +220,21 @@ def test_zero_state(tn):
     g = gpu_mps(tn, arrs)
     g.left_canonise()
     assert all(t.shape == (2, 1, 1) for t in g)
+
+
+@pytest.mark.parametrize("m,n", [(30000, 40), (24577, 9), (40000, 70)])
+def test_tall_qr_and_svd_beyond_register_panel(tn, m, n):
+    """QR / SVD of matrices taller than one register panel (24576 rows):
+    the grid-parallel column path, same Householder sign convention."""
+    rng = np.random.default_rng(m + n)
+    a = rng.standard_normal((m, n))
+    Q, R = tn.tensor_qr(tn.Tensor(a, ["r", "c"]), ["r"])
+    Qo, Ro = o.tensor_qr(o.OTensor(a, ["r", "c"]), ["r"])
+    assert Q.labels == Qo.labels and R.labels == Ro.labels
+    np.testing.assert_allclose(R.numpy(), Ro.data, rtol=1e-9, atol=1e-11 * np.abs(Ro.data).max())
+    np.testing.assert_allclose(Q.numpy(), Qo.data, atol=1e-11)
+    U, S, V = tn.tensor_svd(tn.Tensor(a, ["r", "c"]), ["r"])
+    s_ref = np.linalg.svd(a, compute_uv=False)
+    np.testing.assert_allclose(np.diag(S.numpy()), s_ref, rtol=1e-11)
+    rec = U.numpy() @ S.numpy() @ V.numpy()
+    np.testing.assert_allclose(rec, a, atol=1e-11 * np.abs(a).max())

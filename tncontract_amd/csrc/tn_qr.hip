@@ -419,6 +419,86 @@ size_t qr_layout(int64_t m, int64_t n, char* base, size_t cap, QrWs* ws) {
   return a.used;
 }
 
+// Panels taller than the register panel holds (w = 1): one column per step,
+// three grid launches -- partial sums of squares below the pivot, the
+// reflector (beta, tau, scale; same convention as the register panel) from
+// one block, then the scaling of the reflector below the pivot.  The
+// partials and the scale live in ws.Ypart, which is free between a panel and
+// its update.
+constexpr int QTALL = 4096;  // rows per block of the tall-column kernels
+
+__device__ __forceinline__ double block_sum256(double s, double* red) {
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) qr_tall_sumsq(const double* __restrict__ A, int64_t m,
+                                                      int64_t ld, int64_t c0,
+                                                      double* __restrict__ part) {
+  __shared__ double red[4];
+  const int64_t r0 = c0 + 1 + (int64_t)blockIdx.x * QTALL;
+  const int64_t r1 = r0 + QTALL < m ? r0 + QTALL : m;
+  const double* col = A + c0 * ld;
+  double s = 0.0;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) s = fma(col[r], col[r], s);
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) qr_tall_reflector(double* __restrict__ A, int64_t ld,
+                                                          int64_t c0, double* __restrict__ part,
+                                                          int nchunk, double* __restrict__ tau,
+                                                          double* __restrict__ Tp) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nchunk; i += 256) s += part[i];
+  const double xnorm2 = block_sum256(s, red);
+  if (threadIdx.x == 0) {
+    double* pv = A + c0 + c0 * ld;
+    const double alpha = *pv;
+    double tj = 0.0, scale = 0.0, beta = alpha;
+    if (xnorm2 > 0.0) {
+      const double nrm = sqrt(alpha * alpha + xnorm2);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      tj = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    *pv = beta;
+    tau[c0] = tj;
+    Tp[0] = tj;
+    part[nchunk] = scale;
+  }
+}
+
+__global__ void __launch_bounds__(256) qr_tall_scale(double* __restrict__ A, int64_t m, int64_t ld,
+                                                      int64_t c0, const double* __restrict__ part,
+                                                      int nchunk) {
+  const double scale = part[nchunk];
+  const int64_t r0 = c0 + 1 + (int64_t)blockIdx.x * QTALL;
+  const int64_t r1 = r0 + QTALL < m ? r0 + QTALL : m;
+  double* col = A + c0 * ld;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) col[r] *= scale;
+}
+
+static int run_tall_column(double* A, int64_t m, int64_t ld, int64_t c0, double* tau, double* Tp,
+                           double* part, hipStream_t st) {
+  const int nchunk = (int)ceil_div(std::max<int64_t>(m - c0 - 1, 0), QTALL);
+  if (nchunk > 0) {
+    hipLaunchKernelGGL(qr_tall_sumsq, dim3((unsigned)nchunk), dim3(256), 0, st, A, m, ld, c0, part);
+    TN_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(qr_tall_reflector, dim3(1), dim3(256), 0, st, A, ld, c0, part, nchunk, tau, Tp);
+  TN_CHECK_LAUNCH();
+  if (nchunk > 0) {
+    hipLaunchKernelGGL(qr_tall_scale, dim3((unsigned)nchunk), dim3(256), 0, st, A, m, ld, c0, part,
+                       nchunk);
+    TN_CHECK_LAUNCH();
+  }
+  return TN_OK;
+}
+
 typedef void (*PanelFn)(double*, int64_t, int64_t, int64_t, int, double*, double*);
 
 struct PanelChoice {
@@ -444,7 +524,7 @@ static int panel_width(int64_t mrem) {
 }
 
 static int run_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, double* tau, double* Tp,
-                     hipStream_t st) {
+                     double* part, hipStream_t st) {
   int64_t R = ceil_div(m - c0, PT);
   if (R <= 1) return launch_panel<1, 8>(A, m, ld, c0, w, tau, Tp, st);
   if (R <= 2) return launch_panel<2, 8>(A, m, ld, c0, w, tau, Tp, st);
@@ -455,9 +535,8 @@ static int run_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, double
   if (R <= 14) return launch_panel<14, 4>(A, m, ld, c0, w, tau, Tp, st);
   if (R <= 24) return launch_panel<24, 2>(A, m, ld, c0, w, tau, Tp, st);
   if (R <= 48) return launch_panel<48, 1>(A, m, ld, c0, w, tau, Tp, st);
-  set_error(TN_ERR_UNSUPPORTED, "tn_qr: %lld rows exceed the register-panel limit (24576)",
-            (long long)(m - c0));
-  return TN_ERR_UNSUPPORTED;
+  TN_REQUIRE(w == 1, "tn_qr: tall panel of width %d", w);
+  return run_tall_column(A, m, ld, c0, tau, Tp, part, st);
 }
 
 static int run_apply_panel(double* A, int64_t m, int64_t ld, int64_t c0, int w, const double* Tp,
@@ -492,7 +571,7 @@ int qr_factor(double* W, int64_t m, int64_t n, const QrWs& ws, hipStream_t st) {
     for (int64_t jj = j0; jj < j0 + nbo;) {
       int wmax = panel_width(m - jj);
       int w = (int)std::min<int64_t>(wmax, j0 + nbo - jj);
-      TN_TRY(run_panel(W, m, m, jj, w, ws.tau, ws.Tp, st));
+      TN_TRY(run_panel(W, m, m, jj, w, ws.tau, ws.Tp, ws.Ypart, st));
       TN_TRY(run_apply_panel(W, m, m, jj, w, ws.Tp, jj + w, j0 + nbo, ws.Ypart, st));
       jj += w;
     }
