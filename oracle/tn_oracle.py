@@ -25,6 +25,14 @@ How each function is pinned (checks live in ``tests/test_oracle.py``):
     dense state; closed-form energy of the fully polarised state.
   * inner_product_mps / frob_distance_squared / expvals_mps: dense brute force,
     product-state closed forms.
+  * tensor_lq: L.Q reconstruction, orthonormal Q rows, lower-triangular L.
+  * ladder_contract (left / right / pairwise branches, intermediate
+    contractions) and contract_virtual_indices: products of dense transfer
+    matrices; contract_multi_index_tensor_with_one_dim_array through the
+    square-lattice einsum brute force (tests/test_oracle_square_lattice.py).
+  * variational_compress: exact at sufficient chi, never worse than its
+    svd_compress start, left canonical, and the projection identities
+    <w|v> = <w|w>, |v - w|^2 = |v|^2 - |w|^2 of the final single-site update.
   * reference fixture tests/golden/random_10site_mps.npz (extracted from the
     reference's tests/random_10site_mps.dat without unpickling): the
     invariants of tncontract/tests/test_canonical_form_mps_conversion.py.
