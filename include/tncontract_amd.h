@@ -72,6 +72,17 @@ int tn_axpby(const void* x, const void* y, void* out, int dtype, int64_t n, doub
 /* out (k x k, row-major) = diag(s) (S = np.diag(s) in tensor_svd, tensor.py:940). */
 int tn_diag_embed(const double* s, int64_t k, void* out, int dtype, void* stream);
 
+/* Inverse of a diagonal matrix (Tensor.inv = np.linalg.inv, tensor.py:487-488,
+ * applied to the singular-value matrices of MatrixProductStateCanonical,
+ * onedim_core.py:1079-1082, 1160-1163, 1211-1214, 1876).  out (n x n,
+ * row-major) = diag(1 / A_ii), A read as A[i*rs + j*cs]; info (device, 2
+ * doubles) = {number of nonzero off-diagonal entries, number of zero diagonal
+ * entries} so the caller can detect a non-diagonal or singular A.
+ * ws: >= tn_diag_inverse_workspace(n) bytes. */
+size_t tn_diag_inverse_workspace(int64_t n);
+int tn_diag_inverse(const void* A, int64_t n, int64_t rs, int64_t cs, void* out, int dtype,
+                    double* info, void* ws, size_t ws_bytes, void* stream);
+
 /* *out (device double) = sum x_i^2 in FP64 (np.linalg.norm, tensor.py:590;
  * onedim_core.py:272, 305, 656).  ws: >= tn_sumsq_workspace(n) bytes. */
 size_t tn_sumsq_workspace(int64_t n);

@@ -213,6 +213,35 @@ class OTensor:
     def conjugate(self):
         self.data = self.data.conjugate()
 
+    def inv(self):
+        # tensor.py:487-488
+        self.data = np.linalg.inv(self.data)
+
+    def prime_label(self, labels=None):
+        # tensor.py:181-197 -- prime every label whose unprimed form is one of
+        # `labels` (all labels when None); unprimed form = trailing "'" removed
+        if labels is None:
+            labels = list(self.labels)
+        elif not isinstance(labels, list):
+            labels = [labels]
+        self.labels = [x + "'" if x.rstrip("'") in labels else x for x in self.labels]
+
+    def unprime_label(self, labels=None):
+        # tensor.py:199-226 -- remove one trailing prime from every label whose
+        # unprimed form is one of `labels` (the tensor's labels when None)
+        if labels is None:
+            labels = list(self.labels)
+        elif not isinstance(labels, list):
+            labels = [labels]
+        out = []
+        for x in self.labels:
+            if x.rstrip("'") in labels:
+                if not x.endswith("'"):
+                    raise ValueError("label is not primed")
+                x = x[:-1]
+            out.append(x)
+        self.labels = out
+
     def norm(self):
         return np.linalg.norm(self.data)
 
@@ -751,6 +780,8 @@ class OMPO(_O1D):
 def contract_mps_mpo(mps, mpo):
     """onedim_core.py:1691-1708 -- per-site contraction over phys/physin, then
     consolidate_indices (alphabetical; MPS axis slower than MPO axis)."""
+    if isinstance(mps, OMPSC):
+        raise NotImplementedError("Function not implemented for" + "MatrixProductStateCanonical")
     out = []
     for i in range(len(mps)):
         t = contract(mps[i], mpo[i], mps.phys_label, mpo.physin_label)
@@ -830,8 +861,11 @@ def ladder_contract(array1, array2, label1, label2, start=0, end=None,
 
 
 def inner_product_mps(mps_bra, mps_ket, complex_conjugate_bra=True, return_whole_tensor=False):
-    """onedim_core.py:1666-1683"""
-    t = ladder_contract(mps_bra, mps_ket, mps_bra.phys_label, mps_ket.phys_label,
+    """onedim_core.py:1666-1683 (canonical-form inputs go through their
+    left-canonical form)"""
+    bra = canonical_to_left_canonical(mps_bra) if isinstance(mps_bra, OMPSC) else mps_bra
+    ket = canonical_to_left_canonical(mps_ket) if isinstance(mps_ket, OMPSC) else mps_ket
+    t = ladder_contract(bra, ket, mps_bra.phys_label, mps_ket.phys_label,
                         complex_conjugate_array1=complex_conjugate_bra)
     return t if return_whole_tensor else t.data
 
@@ -939,3 +973,280 @@ def expvals_mps(mps, oplist=[], sites=None, output_label=None, canonised=None):
         mps.reverse()
         out = out[::-1]
     return out
+
+
+# ---------------------------------------------------------------- canonical form (Vidal)
+class OMPSC(_O1D):
+    """onedim_core.py:880-1326 -- Lambda Gamma Lambda ... Gamma Lambda: even
+    positions hold the (diagonal) singular-value matrices, odd positions the
+    Gamma tensors; physical site n sits at position 2n+1."""
+
+    def __init__(self, tensors, left_label="left", right_label="right", phys_label="phys"):
+        super().__init__(tensors, left_label, right_label)
+        self.phys_label = phys_label
+
+    def copy(self):
+        return OMPSC([x.copy() for x in self], self.left_label, self.right_label, self.phys_label)
+
+    def replace_labels(self, old, new):
+        old = old if isinstance(old, list) else [old]
+        new = new if isinstance(new, list) else [new]
+        super().replace_labels(old, new)
+        if self.phys_label in old:
+            self.phys_label = new[old.index(self.phys_label)]
+
+    def standard_labels(self, suffix=""):
+        self.replace_labels([self.left_label, self.right_label, self.phys_label],
+                            ["left" + suffix, "right" + suffix, "phys" + suffix])
+
+    def physical_site(self, n):
+        return 2 * n + 1
+
+    def singular_site(self, n):
+        return 2 * n
+
+    def physdim(self, site):
+        return self.data[self.physical_site(site)].index_dimension(self.phys_label)
+
+    def singulardim(self, site):
+        return self.data[self.singular_site(site)].index_dimension(self.left_label)
+
+    @property
+    def nsites_physical(self):
+        return int((self.nsites - 1) / 2)
+
+    def norm(self, canonical_form=True):
+        """onedim_core.py:988-1002"""
+        if canonical_form is True:
+            return np.linalg.norm(self[-1].data) * np.linalg.norm(self[0].data)
+        return np.sqrt(inner_product_mps(self, self))
+
+    def check_canonical_form(self, threshold=1e-14, print_output=False):
+        """onedim_core.py:1004-1065 -- Lambda.Gamma left isometries and
+        Gamma.Lambda right isometries (edge sites may be unnormalised)."""
+        not_left, not_right, not_norm = [], [], []
+        n = self.nsites_physical
+        for which, bad in (("left", not_left), ("right", not_right)):
+            for i in range(n):
+                p = self.physical_site(i)
+                if which == "left":
+                    A = self[p - 1][self.right_label, ] * self[p][self.left_label, ]
+                    keep = [self.phys_label, self.left_label]
+                else:
+                    A = self[p][self.right_label, ] * self[p + 1][self.left_label, ]
+                    keep = [self.phys_label, self.right_label]
+                I = contract(A, conjugate(A), keep, keep)
+                if np.linalg.norm(I.data - np.identity(I.data.shape[0])) > threshold:
+                    if i == 0 or i == n - 1:
+                        f = I.data.flatten()
+                        if len(f[np.abs(f) > threshold]) > 1:
+                            bad.append(i)
+                        else:
+                            not_norm.append(i)
+                    else:
+                        bad.append(i)
+        return not_left, not_right, not_norm
+
+    def _two_site_update(self, start, U, S, V):
+        S1_inv = self[start].copy()
+        S1_inv.inv()
+        S2_inv = self[start + 4].copy()
+        S2_inv.inv()
+        self[start + 1] = S1_inv[self.right_label, ] * U[self.left_label, ]
+        self[start + 2] = S
+        self[start + 3] = V[self.right_label, ] * S2_inv[self.left_label, ]
+
+    def compress_bond(self, singular_site, chi=None, threshold=1e-15):
+        """onedim_core.py:1067-1095 (the early-exit test reads
+        singulardim(singular_site(n)), i.e. position 4n, as the reference
+        does)."""
+        site = self.singular_site(singular_site)
+        if self.singulardim(site) == 1:
+            return
+        start, end = site - 2, site + 2
+        self[end - 1].prime_label(self.phys_label)
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        S1_inv = self[start].copy()
+        S1_inv.inv()
+        S2_inv = self[end].copy()
+        S2_inv.inv()
+        U, S, V = truncated_svd(t, [self.phys_label, self.left_label], chi=chi,
+                                threshold=threshold, absorb_singular_values=None)
+        U.replace_label("svd_in", self.right_label)
+        V.replace_label("svd_out", self.left_label)
+        S.replace_label(["svd_out", "svd_in"], [self.left_label, self.right_label])
+        self[start + 1] = S1_inv[self.right_label, ] * U[self.left_label, ]
+        self[start + 2] = S
+        self[end - 1] = V[self.right_label, ] * S2_inv[self.left_label, ]
+        self[end - 1].unprime_label(self.phys_label)
+
+    def compress_all(self, chi=None, threshold=1e-15, normalise=False):
+        raise NotImplementedError
+
+    def _gate_labels(self, gate, gate_outputs, gate_inputs):
+        if gate_outputs is None and gate_inputs is None:
+            gate_outputs = gate.labels[:int(len(gate.labels) / 2)]
+            gate_inputs = gate.labels[int(len(gate.labels) / 2):]
+        elif gate_outputs is None:
+            gate_outputs = [x for x in gate.labels if x not in gate_inputs]
+        elif gate_inputs is None:
+            gate_inputs = [x for x in gate.labels if x not in gate_outputs]
+        if len(gate_outputs) != len(gate_inputs):
+            raise ValueError("len(gate_outputs) != len(gate_inputs)")
+        return gate_outputs, gate_inputs
+
+    def apply_gate(self, gate, firstsite, gate_outputs=None, gate_inputs=None, chi=None,
+                   threshold=1e-15):
+        """onedim_core.py:1100-1186 -- contract Lambda Gamma (Lambda Gamma)
+        Lambda, apply the gate, re-split by truncated SVD, divide the outer
+        Lambdas back out."""
+        gate_outputs, gate_inputs = self._gate_labels(gate, gate_outputs, gate_inputs)
+        nsites = len(gate_inputs)
+        if nsites > 2:
+            raise NotImplementedError("gate acting on more than two sites.")
+        start = self.physical_site(firstsite) - 1
+        end = self.physical_site(firstsite + nsites - 1) + 1
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        t = contract(t, gate, self.phys_label, gate_inputs)
+        S1_inv = self[start].copy()
+        S1_inv.inv()
+        S2_inv = self[end].copy()
+        S2_inv.inv()
+        if nsites == 1:
+            t.replace_label([gate_outputs[0]], [self.phys_label])
+            t = S1_inv[self.right_label, ] * t[self.left_label, ]
+            self[start + 1] = t[self.right_label, ] * S2_inv[self.left_label, ]
+        elif nsites == 2:
+            U, S, V = truncated_svd(t, [gate_outputs[0], self.left_label], chi=chi,
+                                    threshold=threshold, absorb_singular_values=None)
+            U.replace_label(["svd_in", gate_outputs[0]], [self.right_label, self.phys_label])
+            V.replace_label(["svd_out", gate_outputs[1]], [self.left_label, self.phys_label])
+            S.replace_label(["svd_out", "svd_in"], [self.left_label, self.right_label])
+            self[start + 1] = S1_inv[self.right_label, ] * U[self.left_label, ]
+            self[start + 2] = S
+            self[start + 3] = V[self.right_label, ] * S2_inv[self.left_label, ]
+
+    def swap_gate(self, i, chi=None, threshold=1e-15):
+        """onedim_core.py:1188-1232 -- swap the physical indices of sites i and
+        i+1 by re-splitting Lambda Gamma Lambda Gamma Lambda with the first
+        site's physical index moved to the right factor."""
+        start = self.physical_site(i) - 1
+        end = self.physical_site(i + 1) + 1
+        self[start + 1].prime_label(self.phys_label)
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        S1_inv = self[start].copy()
+        S1_inv.inv()
+        S2_inv = self[end].copy()
+        S2_inv.inv()
+        U, S, V = truncated_svd(t, [self.left_label, self.phys_label], chi=chi,
+                                threshold=threshold, absorb_singular_values=None)
+        V.unprime_label(self.phys_label)
+        U.replace_label("svd_in", self.right_label)
+        V.replace_label("svd_out", self.left_label)
+        S.replace_label(["svd_out", "svd_in"], [self.left_label, self.right_label])
+        self[start + 1] = S1_inv[self.right_label, ] * U[self.left_label, ]
+        self[start + 2] = S
+        self[start + 3] = V[self.right_label, ] * S2_inv[self.left_label, ]
+
+    def expval(self, gate, firstsite, gate_outputs=None, gate_inputs=None):
+        """onedim_core.py:1234-1290"""
+        gate_outputs, gate_inputs = self._gate_labels(gate, gate_outputs, gate_inputs)
+        start = self.physical_site(firstsite) - 1
+        end = self.physical_site(firstsite + len(gate_inputs) - 1) + 1
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        td = t.copy()
+        td.conjugate()
+        exp = t[self.phys_label, ] * gate[gate_inputs]
+        exp = td[self.phys_label, ] * exp[gate_outputs]
+        exp.tr(self.left_label, self.left_label, index1=0, index2=1)
+        exp.tr(self.right_label, self.right_label, index1=0, index2=1)
+        return exp
+
+    def ptrace(self, firstsite, lastsite=None):
+        """onedim_core.py:1292-1326"""
+        if lastsite is None:
+            lastsite = firstsite
+        start = self.physical_site(firstsite) - 1
+        end = self.physical_site(lastsite) + 1
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        td = t.copy()
+        td.conjugate()
+        for i, l in enumerate(t.labels):
+            if l == self.phys_label:
+                t.labels[i] = l + "_out" + str(i)
+                td.labels[i] = l + "_in" + str(i)
+        return t[self.left_label, self.right_label] * td[self.left_label, self.right_label]
+
+
+def right_canonical_to_canonical(mps, threshold=1e-14):
+    """onedim_core.py:1836-1884 -- left-to-right SVD sweep of a right-canonical
+    MPS keeping singular values > threshold (absolute); Gamma = Lambda^-1 U."""
+    N = mps.nsites
+    S_prev = OTensor([[1.0]], labels=[mps.left_label, mps.right_label])
+    S_prev_inv = S_prev.copy()
+    B = mps[0]
+    tensors = []
+    lab = unique_label()
+    for i in range(N):
+        U, S, V = tensor_svd(B, [mps.phys_label, mps.left_label], svd_label=lab)
+        sv = np.diag(S.data)
+        keep = sv[sv > threshold]
+        S.data = np.diag(keep)
+        U.data = U.data[:, :, 0:len(keep)]
+        V.data = V.data[0:len(keep)]
+        U.replace_label(lab + "in", mps.right_label)
+        V.replace_label(lab + "out", mps.left_label)
+        S.replace_label([lab + "out", lab + "in"], [mps.left_label, mps.right_label])
+        G = S_prev_inv[mps.right_label, ] * U[mps.left_label, ]
+        tensors.append(S_prev)
+        tensors.append(G)
+        if i == N - 1:
+            tensors.append(S)
+        else:
+            V = S[mps.right_label, ] * V[mps.left_label, ]
+            B = V[mps.right_label, ] * mps[i + 1][mps.left_label, ]
+            S_prev = S.copy()
+            S_prev_inv = S_prev.copy()
+            S_prev_inv.inv()
+    return OMPSC(tensors, mps.left_label, mps.right_label, mps.phys_label)
+
+
+def left_canonical_to_canonical(mps, threshold=1e-14):
+    """onedim_core.py:1887-1894"""
+    r = mps.copy()
+    r.reverse()
+    c = right_canonical_to_canonical(r, threshold=threshold)
+    c.reverse()
+    return c
+
+
+def canonical_to_right_canonical(mps):
+    """onedim_core.py:1897-1911"""
+    N = mps.nsites_physical
+    tensors = []
+    for i in range(N - 1):
+        tensors.append(mps[mps.physical_site(i)][mps.right_label, ]
+                       * mps[mps.physical_site(i) + 1][mps.left_label, ])
+    tensors.append(mps[mps.physical_site(N - 1)])
+    tensors[0].data = (tensors[0].data * np.linalg.norm(mps[-1].data)
+                       * np.linalg.norm(mps[0].data))
+    return OMPS(tensors, mps.left_label, mps.right_label, mps.phys_label)
+
+
+def canonical_to_left_canonical(mps):
+    """onedim_core.py:1914-1928"""
+    N = mps.nsites_physical
+    tensors = [mps[mps.physical_site(0)]]
+    for i in range(1, N):
+        tensors.append(mps[mps.physical_site(i) - 1][mps.right_label, ]
+                       * mps[mps.physical_site(i)][mps.left_label, ])
+    tensors[-1].data = (tensors[-1].data * np.linalg.norm(mps[-1].data)
+                        * np.linalg.norm(mps[0].data))
+    return OMPS(tensors, mps.left_label, mps.right_label, mps.phys_label)
+
+
+def canonical_form_mps(orig_mps, chi=0, threshold=1e-14, normalise=False):
+    """onedim_core.py:1448-1454"""
+    mps = orig_mps.copy()
+    mps.right_canonise(chi=chi, threshold=threshold, normalise=normalise)
+    return right_canonical_to_canonical(mps, threshold=threshold)

@@ -74,6 +74,8 @@ def lib():
         "tn_scale": (i32, [vp, i32, i64, dbl, vp]),
         "tn_axpby": (i32, [vp, vp, vp, i32, i64, dbl, dbl, vp]),
         "tn_diag_embed": (i32, [vp, i64, vp, i32, vp]),
+        "tn_diag_inverse_workspace": (sz, [i64]),
+        "tn_diag_inverse": (i32, [vp, i64, i64, i64, vp, i32, vp, vp, sz, vp]),
         "tn_sumsq_workspace": (sz, [i64]),
         "tn_sumsq": (i32, [vp, i32, i64, vp, vp, sz, vp]),
         "tn_trace": (i32, [vp, mp, i64, i64, vp, i32, vp]),
@@ -290,6 +292,19 @@ def axpby(x, y, a, b):
     if x.numel():
         check(lib().tn_axpby(ptr(x), ptr(y), ptr(out), dtype_code(x), x.numel(), float(a), float(b), stream_ptr()))
     return out
+
+
+def diag_inverse(a):
+    """(diag(1 / a_ii) as a dense device matrix, [nonzero off-diagonal count,
+    zero diagonal count]) of a square strided device matrix (synchronises)."""
+    n = a.shape[0]
+    out = torch.empty((n, n), dtype=a.dtype, device=a.device)
+    info = torch.zeros(2, dtype=torch.float64, device=a.device)
+    ws = workspace(lib().tn_diag_inverse_workspace(n))
+    check(lib().tn_diag_inverse(ptr(a) if n else None, n, a.stride(0) if n else 0,
+                                a.stride(1) if n else 0, ptr(out) if n else None, dtype_code(a),
+                                ptr(info), ptr(ws), ws.numel(), stream_ptr()))
+    return out, info.cpu().numpy()
 
 
 def diag_embed(s, dtype=torch.float64):

@@ -159,6 +159,63 @@ __global__ void __launch_bounds__(256) sum_final(const double* __restrict__ part
   if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
 }
 
+// out (n x n, row-major) = diag(1 / A_ii) with A read through its row and
+// column strides; part[2b] / part[2b+1] = this block's count of nonzero
+// off-diagonal entries / zero diagonal entries of A.
+template <typename T>
+__global__ void __launch_bounds__(256) diag_inverse_kernel(const T* __restrict__ A, int64_t n,
+                                                           int64_t rs, int64_t cs,
+                                                           T* __restrict__ out,
+                                                           double* __restrict__ part) {
+  __shared__ double red[2][4];
+  double offd = 0.0, zdiag = 0.0;
+  const int64_t nn = n * n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e - i * n;
+    const T v = A[i * rs + j * cs];
+    if (i == j) {
+      out[e] = (T)1 / v;
+      zdiag += v == (T)0 ? 1.0 : 0.0;
+    } else {
+      out[e] = (T)0;
+      offd += v != (T)0 ? 1.0 : 0.0;
+    }
+  }
+  offd = wave_sum(offd);
+  zdiag = wave_sum(zdiag);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = offd;
+    red[1][threadIdx.x >> 6] = zdiag;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    part[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
+__global__ void __launch_bounds__(256) count_final(const double* __restrict__ part, int nblk,
+                                                   double* __restrict__ info) {
+  __shared__ double red[2][4];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    info[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    info[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) trace_kernel(const T* __restrict__ src, Modes out,
                                                     int64_t nout, int64_t dd, int64_t ds,
@@ -391,6 +448,32 @@ int tn_normalize(void* x, int dtype, int64_t n, double* norm_out, void* ws, size
   else
     hipLaunchKernelGGL(normalize_kernel<float>, dim3(blocks), dim3(256), 0, st, (float*)x, n, ss,
                        norm_out);
+  TN_CHECK_LAUNCH();
+  return TN_OK;
+}
+
+size_t tn_diag_inverse_workspace(int64_t n) {
+  (void)n;
+  return sizeof(double) * 2 * tn::kSumsqBlocks;
+}
+
+int tn_diag_inverse(const void* A, int64_t n, int64_t rs, int64_t cs, void* out, int dtype,
+                    double* info, void* ws, size_t ws_bytes, void* stream) {
+  TN_REQUIRE(n >= 0 && info != nullptr, "tn_diag_inverse: bad arguments");
+  TN_REQUIRE(n == 0 || (A && out), "tn_diag_inverse: null pointer");
+  TN_REQUIRE(ws && ws_bytes >= tn_diag_inverse_workspace(n), "tn_diag_inverse: workspace too small");
+  TN_REQUIRE(dtype == TN_F64 || dtype == TN_F32, "tn_diag_inverse: unsupported dtype %d", dtype);
+  hipStream_t st = as_stream(stream);
+  double* part = (double*)ws;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n * n, 256), tn::kSumsqBlocks));
+  if (dtype == TN_F64)
+    hipLaunchKernelGGL(diag_inverse_kernel<double>, dim3(blocks), dim3(256), 0, st,
+                       (const double*)A, n, rs, cs, (double*)out, part);
+  else
+    hipLaunchKernelGGL(diag_inverse_kernel<float>, dim3(blocks), dim3(256), 0, st,
+                       (const float*)A, n, rs, cs, (float*)out, part);
+  TN_CHECK_LAUNCH();
+  hipLaunchKernelGGL(count_final, dim3(1), dim3(256), 0, st, part, blocks, info);
   TN_CHECK_LAUNCH();
   return TN_OK;
 }

@@ -21,7 +21,9 @@ __all__ = ["MatrixProductState", "MatrixProductOperator", "OneDimensionalTensorN
            "inner_product_mps", "ladder_contract", "left_canonical_form_mps",
            "mps_complex_conjugate", "reverse_mps", "right_canonical_form_mps",
            "svd_compress_mps", "variational_compress_mps", "contract_virtual_indices",
-           "contract_multi_index_tensor_with_one_dim_array"]
+           "contract_multi_index_tensor_with_one_dim_array", "MatrixProductStateCanonical",
+           "right_canonical_to_canonical", "left_canonical_to_canonical",
+           "canonical_to_right_canonical", "canonical_to_left_canonical", "canonical_form_mps"]
 
 
 class OneDimensionalTensorNetwork:
@@ -285,6 +287,228 @@ def _dist_identity(I):
     return float(np.sqrt(L.sumsq(L.axpby(L.contiguous(I.data), eye, 1.0, -1.0))))
 
 
+class MatrixProductStateCanonical(OneDimensionalTensorNetwork):
+    """onedim_core.py:880-1326: Lambda Gamma Lambda ... Gamma Lambda -- even
+    positions hold the diagonal singular-value matrices, odd positions the
+    Gamma tensors (physical site n at position 2n+1).  Two-site updates
+    contract Lambda Gamma Lambda Gamma Lambda on the GPU, re-split it by the
+    device truncated SVD and divide the outer Lambdas back out (exact
+    diagonal inverses on the device)."""
+
+    def __init__(self, tensors, left_label="left", right_label="right", phys_label="phys"):
+        OneDimensionalTensorNetwork.__init__(self, tensors, left_label=left_label,
+                                             right_label=right_label)
+        self.phys_label = phys_label
+
+    def __repr__(self):
+        return ("MatrixProductStateCanonical(tensors=%r, left_label=%r,right_label=%r, "
+                "phys_label=%r)" % (self.data, self.left_label, self.right_label,
+                                    self.phys_label))
+
+    def __str__(self):
+        return ("MatrixProductStateCanonical object: " + "sites (incl. singular value sites)= "
+                + str(len(self)) + ", left_label = " + self.left_label + ", right_label = "
+                + self.right_label + ", phys_label = " + self.phys_label)
+
+    def copy(self):
+        return MatrixProductStateCanonical([x.copy() for x in self], self.left_label,
+                                           self.right_label, self.phys_label)
+
+    def replace_labels(self, old_labels, new_labels):
+        if not isinstance(old_labels, list):
+            old_labels = [old_labels]
+        if not isinstance(new_labels, list):
+            new_labels = [new_labels]
+        OneDimensionalTensorNetwork.replace_labels(self, old_labels, new_labels)
+        if self.phys_label in old_labels:
+            self.phys_label = new_labels[old_labels.index(self.phys_label)]
+
+    def standard_labels(self, suffix=""):
+        self.replace_labels([self.left_label, self.right_label, self.phys_label],
+                            ["left" + suffix, "right" + suffix, "phys" + suffix])
+
+    def physical_site(self, n):
+        return 2 * n + 1
+
+    def singular_site(self, n):
+        return 2 * n
+
+    def physdim(self, site):
+        return self.data[self.physical_site(site)].index_dimension(self.phys_label)
+
+    def singulardim(self, site):
+        return self.data[self.singular_site(site)].index_dimension(self.left_label)
+
+    def bonddims(self):
+        return OneDimensionalTensorNetwork.bonddims(self)
+
+    @property
+    def nsites_physical(self):
+        return int((self.nsites - 1) / 2)
+
+    def norm(self, canonical_form=True):
+        """onedim_core.py:988-1002"""
+        if canonical_form is True:
+            return self[-1].norm() * self[0].norm()
+        return float(np.sqrt(inner_product_mps(self, self)))
+
+    def check_canonical_form(self, threshold=1e-14, print_output=True):
+        """onedim_core.py:1004-1065: Lambda.Gamma left isometries, Gamma.Lambda
+        right isometries (identity residuals reduced on the GPU)."""
+        not_left, not_right, not_normalised = [], [], []
+        n = self.nsites_physical
+        for side in ("left", "right"):
+            bad = not_left if side == "left" else not_right
+            for i in range(n):
+                p = self.physical_site(i)
+                if side == "left":
+                    A = self[p - 1][self.right_label, ] * self[p][self.left_label, ]
+                    keep = [self.phys_label, self.left_label]
+                else:
+                    A = self[p][self.right_label, ] * self[p + 1][self.left_label, ]
+                    keep = [self.phys_label, self.right_label]
+                I = tsr.contract(A, tsr.conjugate(A), keep, keep)
+                if _dist_identity(I) > threshold:
+                    if i == 0 or i == n - 1:
+                        f = I.numpy().flatten()
+                        if len(f[np.abs(f) > threshold]) > 1:
+                            bad.append(i)
+                        else:
+                            not_normalised.append(i)
+                    else:
+                        bad.append(i)
+        if print_output:
+            if len(not_left) == 0 and len(not_right) == 0:
+                if len(not_normalised) == 0:
+                    print("MPS in canonical form (normalised)")
+                else:
+                    print("MPS in canonical form (unnormalised)")
+            else:
+                print("Physical sites not left-canonical:")
+                print(not_left)
+                print("Physical sites not right-canonical:")
+                print(not_right)
+        return not_left, not_right, not_normalised
+
+    def _inverse(self, pos):
+        t = self[pos].copy()
+        t.inv()
+        return t
+
+    def compress_bond(self, singular_site, chi=None, threshold=1e-15):
+        """onedim_core.py:1067-1095.  As in the reference, the early-exit test
+        reads singulardim(singular_site(n)), i.e. position 4n."""
+        site = self.singular_site(singular_site)
+        if self.singulardim(site) == 1:
+            return
+        start, end = site - 2, site + 2
+        self[end - 1].prime_label(self.phys_label)
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        S1_inv = self._inverse(start)
+        S2_inv = self._inverse(end)
+        U, S, V = tsr.truncated_svd(t, [self.phys_label, self.left_label], chi=chi,
+                                    threshold=threshold, absorb_singular_values=None)
+        U.replace_label("svd_in", self.right_label)
+        V.replace_label("svd_out", self.left_label)
+        S.replace_label(["svd_out", "svd_in"], [self.left_label, self.right_label])
+        self[start + 1] = S1_inv[self.right_label, ] * U[self.left_label, ]
+        self[start + 2] = S
+        self[end - 1] = V[self.right_label, ] * S2_inv[self.left_label, ]
+        self[end - 1].unprime_label(self.phys_label)
+
+    def compress_all(self, chi=None, threshold=1e-15, normalise=False):
+        raise NotImplementedError
+
+    def _gate_labels(self, gate, gate_outputs, gate_inputs):
+        if gate_outputs is None and gate_inputs is None:
+            gate_outputs = gate.labels[:int(len(gate.labels) / 2)]
+            gate_inputs = gate.labels[int(len(gate.labels) / 2):]
+        elif gate_outputs is None:
+            gate_outputs = [x for x in gate.labels if x not in gate_inputs]
+        elif gate_inputs is None:
+            gate_inputs = [x for x in gate.labels if x not in gate_outputs]
+        if len(gate_outputs) != len(gate_inputs):
+            raise ValueError("len(gate_outputs) != len(gate_inputs)")
+        return gate_outputs, gate_inputs
+
+    def apply_gate(self, gate, firstsite, gate_outputs=None, gate_inputs=None, chi=None,
+                   threshold=1e-15):
+        """onedim_core.py:1100-1186: 1- and 2-site gates, optimal truncation
+        at the updated bond (exact SVD of the gated two-site block)."""
+        gate_outputs, gate_inputs = self._gate_labels(gate, gate_outputs, gate_inputs)
+        nsites = len(gate_inputs)
+        if nsites > 2:
+            raise NotImplementedError("gate acting on more than two sites.")
+        start = self.physical_site(firstsite) - 1
+        end = self.physical_site(firstsite + nsites - 1) + 1
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        t = tsr.contract(t, gate, self.phys_label, gate_inputs)
+        S1_inv = self._inverse(start)
+        S2_inv = self._inverse(end)
+        if nsites == 1:
+            t.replace_label([gate_outputs[0]], [self.phys_label])
+            t = S1_inv[self.right_label, ] * t[self.left_label, ]
+            self[start + 1] = t[self.right_label, ] * S2_inv[self.left_label, ]
+        elif nsites == 2:
+            U, S, V = tsr.truncated_svd(t, [gate_outputs[0], self.left_label], chi=chi,
+                                        threshold=threshold, absorb_singular_values=None)
+            U.replace_label(["svd_in", gate_outputs[0]], [self.right_label, self.phys_label])
+            V.replace_label(["svd_out", gate_outputs[1]], [self.left_label, self.phys_label])
+            S.replace_label(["svd_out", "svd_in"], [self.left_label, self.right_label])
+            self[start + 1] = S1_inv[self.right_label, ] * U[self.left_label, ]
+            self[start + 2] = S
+            self[start + 3] = V[self.right_label, ] * S2_inv[self.left_label, ]
+
+    def swap_gate(self, i, chi=None, threshold=1e-15):
+        """onedim_core.py:1188-1232: swap the physical indices of sites i and
+        i+1 by re-splitting the two-site block with the first site's
+        (primed) physical index moved to the right factor."""
+        start = self.physical_site(i) - 1
+        end = self.physical_site(i + 1) + 1
+        self[start + 1].prime_label(self.phys_label)
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        S1_inv = self._inverse(start)
+        S2_inv = self._inverse(end)
+        U, S, V = tsr.truncated_svd(t, [self.left_label, self.phys_label], chi=chi,
+                                    threshold=threshold, absorb_singular_values=None)
+        V.unprime_label(self.phys_label)
+        U.replace_label("svd_in", self.right_label)
+        V.replace_label("svd_out", self.left_label)
+        S.replace_label(["svd_out", "svd_in"], [self.left_label, self.right_label])
+        self[start + 1] = S1_inv[self.right_label, ] * U[self.left_label, ]
+        self[start + 2] = S
+        self[start + 3] = V[self.right_label, ] * S2_inv[self.left_label, ]
+
+    def expval(self, gate, firstsite, gate_outputs=None, gate_inputs=None):
+        """onedim_core.py:1234-1290"""
+        gate_outputs, gate_inputs = self._gate_labels(gate, gate_outputs, gate_inputs)
+        start = self.physical_site(firstsite) - 1
+        end = self.physical_site(firstsite + len(gate_inputs) - 1) + 1
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        td = t.copy()
+        td.conjugate()
+        exp = t[self.phys_label, ] * gate[gate_inputs]
+        exp = td[self.phys_label, ] * exp[gate_outputs]
+        exp.tr(self.left_label, self.left_label, index1=0, index2=1)
+        exp.tr(self.right_label, self.right_label, index1=0, index2=1)
+        return exp
+
+    def ptrace(self, firstsite, lastsite=None):
+        """onedim_core.py:1292-1326"""
+        if lastsite is None:
+            lastsite = firstsite
+        start = self.physical_site(firstsite) - 1
+        end = self.physical_site(lastsite) + 1
+        t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+        td = t.copy()
+        td.conjugate()
+        for i, l in enumerate(t.labels):
+            if l == self.phys_label:
+                t.labels[i] = l + "_out" + str(i)
+                td.labels[i] = l + "_in" + str(i)
+        return t[self.left_label, self.right_label] * td[self.left_label, self.right_label]
+
+
 class MatrixProductOperator(OneDimensionalTensorNetwork):
     """onedim_core.py:1329-1367"""
 
@@ -513,9 +737,96 @@ def ladder_contract(array1, array2, label1, label2, start=0, end=None,
     return inter if return_intermediate_contractions else C
 
 
+def right_canonical_to_canonical(mps, threshold=1e-14):
+    """onedim_core.py:1836-1884: left-to-right SVD sweep of a right-canonical
+    MPS keeping singular values > threshold; Gamma_i = Lambda_i^-1 U_i."""
+    N = mps.nsites
+    S_prev = tsr.Tensor([[1.0]], labels=[mps.left_label, mps.right_label])
+    S_prev_inv = S_prev.copy()
+    B = mps[0]
+    tensors = []
+    svd_label = unique_label()
+    for i in range(N):
+        U, S, V, s = tsr._tensor_svd(B, [mps.phys_label, mps.left_label], svd_label=svd_label)
+        sv = s.cpu().numpy()
+        k = int(np.count_nonzero(sv > threshold))
+        S = tsr.Tensor._wrap(L.diag_embed(_upload(sv[:k]), U.dtype), S.labels)
+        U._data = U.data[:, :, 0:k]
+        V._data = V.data[0:k]
+        U.replace_label(svd_label + "in", mps.right_label)
+        V.replace_label(svd_label + "out", mps.left_label)
+        S.replace_label([svd_label + "out", svd_label + "in"], [mps.left_label, mps.right_label])
+        G = S_prev_inv[mps.right_label, ] * U[mps.left_label, ]
+        tensors.append(S_prev)
+        tensors.append(G)
+        if i == N - 1:
+            tensors.append(S)
+        else:
+            V = S[mps.right_label, ] * V[mps.left_label, ]
+            B = V[mps.right_label, ] * mps[i + 1][mps.left_label, ]
+            S_prev = S.copy()
+            S_prev_inv = S_prev.copy()
+            S_prev_inv.inv()
+    return MatrixProductStateCanonical(tensors, left_label=mps.left_label,
+                                       right_label=mps.right_label, phys_label=mps.phys_label)
+
+
+def left_canonical_to_canonical(mps, threshold=1e-14):
+    """onedim_core.py:1887-1894"""
+    mpsr = reverse_mps(mps)
+    mpsc = right_canonical_to_canonical(mpsr, threshold=threshold)
+    mpsc.reverse()
+    return mpsc
+
+
+def _rescale_shared(t, factor):
+    """t's data replaced by factor * data (the reference rebinds .data on the
+    tensor object, which may be shared with the source network)."""
+    c = t.copy()
+    c.scale_(factor)
+    t._data = c._data
+
+
+def canonical_to_right_canonical(mps):
+    """onedim_core.py:1897-1911"""
+    N = mps.nsites_physical
+    tensors = []
+    for i in range(N - 1):
+        tensors.append(mps[mps.physical_site(i)][mps.right_label, ]
+                       * mps[mps.physical_site(i) + 1][mps.left_label, ])
+    tensors.append(mps[mps.physical_site(N - 1)])
+    _rescale_shared(tensors[0], mps[-1].norm() * mps[0].norm())
+    return MatrixProductState(tensors, left_label=mps.left_label, right_label=mps.right_label,
+                              phys_label=mps.phys_label)
+
+
+def canonical_to_left_canonical(mps):
+    """onedim_core.py:1914-1928"""
+    N = mps.nsites_physical
+    tensors = [mps[mps.physical_site(0)]]
+    for i in range(1, N):
+        tensors.append(mps[mps.physical_site(i) - 1][mps.right_label, ]
+                       * mps[mps.physical_site(i)][mps.left_label, ])
+    _rescale_shared(tensors[-1], mps[-1].norm() * mps[0].norm())
+    return MatrixProductState(tensors, left_label=mps.left_label, right_label=mps.right_label,
+                              phys_label=mps.phys_label)
+
+
+def canonical_form_mps(orig_mps, chi=0, threshold=1e-14, normalise=False):
+    """onedim_core.py:1448-1454"""
+    mps = orig_mps.copy()
+    mps.right_canonise(chi=chi, threshold=threshold, normalise=normalise)
+    return right_canonical_to_canonical(mps, threshold=threshold)
+
+
 def inner_product_mps(mps_bra, mps_ket, complex_conjugate_bra=True, return_whole_tensor=False):
-    """onedim_core.py:1666-1683 -- FP64 transfer-matrix sweep on the GPU."""
-    t = ladder_contract(mps_bra, mps_ket, mps_bra.phys_label, mps_ket.phys_label,
+    """onedim_core.py:1666-1683 -- FP64 transfer-matrix sweep on the GPU
+    (canonical-form inputs through their left-canonical form)."""
+    bra = canonical_to_left_canonical(mps_bra) \
+        if isinstance(mps_bra, MatrixProductStateCanonical) else mps_bra
+    ket = canonical_to_left_canonical(mps_ket) \
+        if isinstance(mps_ket, MatrixProductStateCanonical) else mps_ket
+    t = ladder_contract(bra, ket, mps_bra.phys_label, mps_ket.phys_label,
                         complex_conjugate_array1=complex_conjugate_bra)
     if return_whole_tensor:
         return t
@@ -533,6 +844,8 @@ def contract_mps_mpo(mps, mpo):
     equal labels fused with the MPS axis slower) is fused into the store of
     the contraction kernel: each site is written once, already in its final
     (left, physout, right) layout."""
+    if isinstance(mps, MatrixProductStateCanonical):
+        raise NotImplementedError("Function not implemented for" + "MatrixProductStateCanonical")
     N = len(mps)
     new_mps = []
     for i in range(N):

@@ -336,6 +336,33 @@ class Tensor:
         L.scale_(self._data, alpha)
         return self
 
+    def inv(self):
+        """Matrix inverse of a 2-index tensor (tensor.py:487-488,
+        np.linalg.inv).  A diagonal matrix (the singular-value matrices of
+        MatrixProductStateCanonical) is inverted exactly on the device,
+        1 / A_ii as LAPACK gives it; any other square matrix through the
+        device SVD, V diag(1/s) U^T.  Singular input raises LinAlgError."""
+        d = self._data
+        if d.dim() != 2 or d.shape[0] != d.shape[1]:
+            raise np.linalg.LinAlgError("Last 2 dimensions of the array must be square")
+        n = d.shape[0]
+        if n == 0:
+            return
+        out, info = L.diag_inverse(d)
+        if info[0] == 0:
+            if info[1] > 0:
+                raise np.linalg.LinAlgError("Singular matrix")
+            self._data = out
+            return
+        src = d if d.dtype == torch.float64 else L.convert(L.contiguous(d), torch.float64)
+        u, s, vt = _svd_device(src, 1)
+        sv = s.cpu().numpy()
+        if sv[-1] == 0.0:
+            raise np.linalg.LinAlgError("Singular matrix")
+        w = L.diag_embed(_to_device(np.ascontiguousarray(1.0 / sv, dtype=np.float64)))
+        inv = _contract_data(vt, [0], _contract_data(w, [1], u, [1]), [0])
+        self._data = inv if d.dtype == torch.float64 else L.convert(inv, d.dtype)
+
 
 class ToContract:
     """``A["a","b"] * B["c","d"]`` shorthand (tensor.py:593-613)."""
