@@ -61,12 +61,15 @@ def parse():
     p.add_argument("--traffic", type=float, default=None,
                    help="HBM bytes per launch of the dominant kernel, from the calibrated "
                         "FETCH_SIZE + WRITE_SIZE counters (reported as roofline.traffic)")
-    p.add_argument("--workload", default="chain", choices=["chain", "batched", "variational", "peps"],
+    p.add_argument("--workload", default="chain", choices=["chain", "batched", "variational", "peps", "tebd"],
                    help="chain: the MPO.MPS compress sweep (default, the headline); batched: "
                         "1024 uneven contraction + truncation jobs (chi ~ U{16..512}, d=2, "
                         "FP32 data, keep sigma/sigma_0 > 1e-8); variational: iterations/s of "
                         "variational_compress on the MPO-applied chain; peps: boundary-MPS "
-                        "inner_product_peps contractions/s")
+                        "inner_product_peps contractions/s; tebd: canonical-MPS two-site "
+                        "gates/s")
+    p.add_argument("--tebd-N", type=int, default=64)
+    p.add_argument("--tebd-chi", type=int, default=128)
     p.add_argument("--peps-L", type=int, default=10)
     p.add_argument("--peps-D", type=int, default=3)
     p.add_argument("--peps-chi", type=int, default=64)
@@ -216,22 +219,24 @@ def batched_workload(args):
     print(json.dumps(rec), flush=True)
 
 
-def roofline_from_prof(prof, traffic=None):
-    """Roofline record for the tag with the largest measured time."""
+def roofline_from_prof(prof, traffic=None, measured=None):
+    """Roofline record for the tag with the largest measured time; `measured`
+    maps kernel tags to PMC-measured HBM bytes per launch for this workload."""
+    measured = measured or {}
     dom = max(prof, key=lambda t: prof[t][0])
     ms, n, fl, by = prof[dom]
     if dom in PEAK_TFLOPS and fl > 0:
         ach = fl / (ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_TFLOPS[dom],
                 "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dom],
-                "traffic": traffic if traffic is not None else MEASURED_TRAFFIC.get(dom),
+                "traffic": traffic if traffic is not None else measured.get(dom),
                 "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n,
                 "flops_per_launch": fl / max(n, 1)}
     else:
         ach = by / (ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": 8000.0, "unit": "GB/s",
                 "frac": ach / 8000.0,
-                "traffic": traffic if traffic is not None else MEASURED_TRAFFIC.get(dom),
+                "traffic": traffic if traffic is not None else measured.get(dom),
                 "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n,
                 "bytes_per_launch": by / max(n, 1)}
     if dom in NOTES:
@@ -426,6 +431,93 @@ def peps_workload(args):
     print(json.dumps(rec), flush=True)
 
 
+def xxz_bond_gate(tau, Jz=1.0, Jxy=1.0):
+    """exp(-tau h) of the XXZ bond Hamiltonian h = Jz Sz Sz + Jxy/2 (S+S- +
+    S-S+) (spin 1/2), as a (o0, o1, i0, i1) array."""
+    import numpy as np
+    sz = np.diag([0.5, -0.5])
+    sp = np.array([[0.0, 1.0], [0.0, 0.0]])
+    h = Jz * np.kron(sz, sz) + 0.5 * Jxy * (np.kron(sp, sp.T) + np.kron(sp.T, sp))
+    w, v = np.linalg.eigh(h)
+    return ((v * np.exp(-tau * w)) @ v.T).reshape(2, 2, 2, 2)
+
+
+def tebd_workload(args):
+    """Two-site gates/s of MatrixProductStateCanonical.apply_gate
+    (onedim_core.py:1100-1186): one step = one imaginary-time XXZ Trotter
+    sweep (every even bond, then every odd bond) of a canonical chain,
+    truncating each updated bond to chi."""
+    import numpy as np
+    import torch
+    import tncontract_amd as tn
+    from tncontract_amd import _lib as L
+    from tncontract_amd.onedim import random_mps_arrays
+    torch.cuda.set_device(0)
+    N, d, chi = args.tebd_N, 2, args.tebd_chi
+    arrs = random_mps_arrays(N, d, chi, seed=args.seed)
+    G = xxz_bond_gate(0.05)
+    gate = tn.Tensor(G, ["o0", "o1", "i0", "i1"])
+    base = tn.onedim.canonical_form_mps(
+        tn.onedim.MatrixProductState([tn.Tensor(a, ["phys", "left", "right"]) for a in arrs]),
+        threshold=1e-14, normalise=True)
+    bonds = list(range(0, N - 1, 2)) + list(range(1, N - 1, 2))
+    torch.cuda.synchronize()
+
+    def step(c):
+        for k in bonds:
+            c.apply_gate(gate, k, chi=chi, threshold=1e-14)
+        return c
+
+    log("tebd: N=%d chi=%d, %d gates per sweep" % (N, chi, len(bonds)))
+    for _ in range(args.warmup):
+        step(base.copy())
+    torch.cuda.synchronize()
+    runs = [base.copy() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in runs:
+        step(c)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    L.prof_collect("gemm_f64", reset=True)
+    L.prof_enable(True)
+    step(base.copy())
+    torch.cuda.synchronize()
+    L.prof_enable(False)
+    prof = {tag: L.prof_collect(tag) for tag in L.PROF_TAGS}
+    L.prof_collect("gemm_f64", reset=True)
+    roof = roofline_from_prof(prof, args.traffic)
+    cpu, lam_err = None, None
+    if not args.no_cpu:
+        from oracle import tn_oracle as o
+        log("cpu baseline: oracle on the same sweep")
+        oc = o.OMPSC([o.OTensor(t.numpy(), list(t.labels)) for t in base], base.left_label,
+                     base.right_label, base.phys_label)
+        og = o.OTensor(G, ["o0", "o1", "i0", "i1"])
+        t1 = time.perf_counter()
+        for k in bonds:
+            oc.apply_gate(og, k, chi=chi, threshold=1e-14)
+        tc = time.perf_counter() - t1
+        mid = runs[0].singular_site(N // 2)
+        lg, lr = np.diag(runs[0][mid].numpy()), np.diag(oc[mid].data)
+        lam_err = float(np.max(np.abs(lg - lr)) / lr[0]) if len(lg) == len(lr) else float("nan")
+        cpu = {"value": len(bonds) / tc, "unit": "gates/s", "cores": len(os.sched_getaffinity(0)),
+               "kind": "port", "sample": "NumPy FP64 oracle, the same Trotter sweep once "
+                                         "(%.2f s)" % tc}
+    rec = {"metric": "MatrixProductStateCanonical.apply_gate two-site gates/s", "value":
+           len(bonds) / dt, "unit": "gates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded random MPS in canonical form, normalised)",
+           "config": {"workload": "imaginary-time XXZ Trotter sweep (tau=0.05, even then odd "
+                                  "bonds), canonical MPS N=%d d=2 chi=%d, apply_gate with chi=%d"
+                                  % (N, chi, chi)},
+           "mid_bond_lambda_rel_err_vs_oracle": lam_err,
+           "kernel_ms_profiled_step": {t: round(v[0], 3) for t, v in prof.items() if v[1]},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -443,6 +535,9 @@ def main():
         return
     if args.workload == "peps":
         peps_workload(args)
+        return
+    if args.workload == "tebd":
+        tebd_workload(args)
         return
     import numpy as np
     import torch
@@ -489,7 +584,7 @@ def main():
     L.prof_enable(False)
     prof = {tag: L.prof_collect(tag) for tag in L.PROF_TAGS}
     L.prof_collect("gemm_f64", reset=True)
-    roof = roofline_from_prof(prof, args.traffic)
+    roof = roofline_from_prof(prof, args.traffic, MEASURED_TRAFFIC)
     kernel_ms = {t: round(v[0], 3) for t, v in prof.items() if v[1]}
 
     # ---- plain FP64 path (the denominator) and the error between the two
