@@ -189,6 +189,24 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
                         int max_sweeps, int* kept_out, int* sweeps_out, void* ws, size_t ws_bytes,
                         void* stream);
 
+/* Many independent FP64 thin SVDs in shared launches: np.linalg.svd
+ * (tensor.py:915) of each job's A (m x n, FP64, row-major), A = U S Vt with S
+ * descending; the same batched one-sided block Jacobi (FP64 rotations),
+ * orthogonality to L eps (L = max(m, n)).  All min(m, n) singular triplets
+ * are returned; the caller truncates.  No counterpart as a batch in the
+ * reference: it serves a layer of TEBD gates on disjoint bonds
+ * (onedim_core.py:1100-1186), whose SVDs are independent. */
+typedef struct tn_svd_job {
+  const double* A; /* m x n FP64 row-major */
+  int64_t m, n;
+  double* U;       /* m x min(m,n) FP64 row-major */
+  double* S;       /* min(m,n), descending */
+  double* Vt;      /* min(m,n) x n FP64 row-major */
+} tn_svd_job;
+size_t tn_batched_svd_workspace(const tn_svd_job* jobs, int njobs);
+int tn_batched_svd(const tn_svd_job* jobs, int njobs, int max_sweeps, int* sweeps_out, void* ws,
+                   size_t ws_bytes, void* stream);
+
 /* ---- learned-rank truncation (no reference counterpart: the reference
  * always runs the full SVD at tensor.py:915) -------------------------------
  * Rank-prediction MLP 34 -> 64 -> 64 -> 1 (ReLU, FP32).  Parameters are one

@@ -58,7 +58,8 @@ def same_lambdas(g, r):
     assert g.bonddims() == r.bonddims()
     assert [t.labels for t in g] == [t.labels for t in r]
     for p in range(0, g.nsites, 2):
-        lg, lr = np.diag(g[p].numpy()), np.diag(r[p].data)
+        lg = np.diag(g[p].numpy())
+        lr = np.diag(r[p].numpy() if hasattr(r[p], "numpy") else r[p].data)
         np.testing.assert_allclose(lg, lr, rtol=1e-10, atol=1e-12 * max(abs(lr).max(), 1e-300))
 
 
@@ -164,3 +165,46 @@ def test_expval_and_ptrace_match_oracle(tn):
     rg, rr = g.ptrace(1, 3), r.ptrace(1, 3)
     assert rg.labels == rr.labels
     np.testing.assert_allclose(rg.numpy(), rr.data, rtol=1e-10, atol=1e-12 * np.abs(rr.data).max())
+
+
+def test_batched_svd_fp64_matches_lapack(tn):
+    from tncontract_amd.batched import batched_svd
+    rng = np.random.default_rng(9)
+    shapes = [(40, 40), (100, 30), (30, 100), (256, 256), (7, 3), (3, 7), (1, 5), (64, 200)]
+    mats = [rng.standard_normal(s) for s in shapes]
+    mats[3][:, 100:] = mats[3][:, :156] @ rng.standard_normal((156, 156))  # rank-deficient
+    U, S, Vt, sweeps = batched_svd([torch.from_numpy(a).cuda() for a in mats])
+    for a, u, s, vt in zip(mats, U, S, Vt):
+        u, s, vt = u.cpu().numpy(), s.cpu().numpy(), vt.cpu().numpy()
+        ref = np.linalg.svd(a, compute_uv=False)
+        np.testing.assert_allclose(s, ref, rtol=1e-11, atol=1e-12 * ref[0])
+        np.testing.assert_allclose((u * s) @ vt, a, atol=1e-11 * ref[0])
+        r = int(np.count_nonzero(ref > 1e-10 * ref[0]))
+        np.testing.assert_allclose(u[:, :r].T @ u[:, :r], np.eye(r), atol=1e-11)
+        np.testing.assert_allclose(vt[:r] @ vt[:r].T, np.eye(r), atol=1e-11)
+    assert np.all(sweeps >= 1)
+
+
+@pytest.mark.parametrize("chi", [None, 5])
+def test_apply_gate_layer_matches_sequential_and_oracle(tn, chi):
+    g, r = both_canonical(tn, 10, 2, 8, seed=11)
+    seq = g.copy()
+    rng = np.random.default_rng(12)
+    G = [rng.standard_normal((2, 2, 2, 2)) for _ in range(5)]
+    ks = [0, 2, 4, 6, 8]
+    g.apply_gate_layer([tn.Tensor(x, ["o0", "o1", "i0", "i1"]) for x in G], ks, chi=chi)
+    for x, k in zip(G, ks):
+        seq.apply_gate(tn.Tensor(x, ["o0", "o1", "i0", "i1"]), k, chi=chi)
+        r.apply_gate(o.OTensor(x, ["o0", "o1", "i0", "i1"]), k, chi=chi)
+    same_lambdas(g, r)
+    same_lambdas(g, seq)
+    same_state(g, r)
+    # odd layer with one shared gate, then the canonical form and the oracle again
+    one = rng.standard_normal((2, 2, 2, 2))
+    g.apply_gate_layer(tn.Tensor(one, ["a", "b", "c", "d"]), [1, 3, 5, 7], chi=chi)
+    for k in (1, 3, 5, 7):
+        r.apply_gate(o.OTensor(one, ["a", "b", "c", "d"]), k, chi=chi)
+    same_lambdas(g, r)
+    same_state(g, r)
+    with pytest.raises(ValueError):
+        g.apply_gate_layer(tn.Tensor(one, ["a", "b", "c", "d"]), [2, 3])

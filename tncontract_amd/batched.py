@@ -24,9 +24,20 @@ class tn_trunc_job(ctypes.Structure):
                 ("S", ctypes.c_void_p), ("Vt", ctypes.c_void_p)]
 
 
+class tn_svd_job(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_void_p), ("m", ctypes.c_int64), ("n", ctypes.c_int64),
+                ("U", ctypes.c_void_p), ("S", ctypes.c_void_p), ("Vt", ctypes.c_void_p)]
+
+
 def _bind():
     lib = L.lib()
     if not hasattr(lib, "_tn_batched_bound"):
+        sp = ctypes.POINTER(tn_svd_job)
+        lib.tn_batched_svd_workspace.restype = ctypes.c_size_t
+        lib.tn_batched_svd_workspace.argtypes = [sp, ctypes.c_int]
+        lib.tn_batched_svd.restype = ctypes.c_int
+        lib.tn_batched_svd.argtypes = [sp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         jp = ctypes.POINTER(tn_trunc_job)
         lib.tn_batched_truncate_workspace.restype = ctypes.c_size_t
         lib.tn_batched_truncate_workspace.argtypes = [jp, ctypes.c_int]
@@ -70,6 +81,35 @@ def batched_truncate(lefts, rights, rel_threshold=1e-8, chi=0, max_sweeps=40):
                                     kept, sweeps, L.ptr(ws), ws.numel(), L.stream_ptr()))
     return ([o[0] for o in outs], [o[1] for o in outs], [o[2] for o in outs],
             np.array(kept[:]), np.array(sweeps[:]))
+
+
+def batched_svd(mats, max_sweeps=40):
+    """Thin FP64 SVDs of many device matrices (np.linalg.svd per matrix,
+    tensor.py:915) in shared launches.  mats[j]: (m_j, n_j) device tensor.
+    Returns (U, S, Vt, sweeps): per-job device factors (S descending) and the
+    host array of Jacobi sweeps."""
+    lib = _bind()
+    nj = len(mats)
+    jobs = (tn_svd_job * nj)()
+    outs, keep_alive = [], []
+    for j, a in enumerate(mats):
+        a = L.contiguous(L.convert(a, torch.float64))
+        m, n = a.shape
+        r = min(m, n)
+        U = torch.empty((m, r), dtype=torch.float64, device=a.device)
+        S = torch.empty((r,), dtype=torch.float64, device=a.device)
+        Vt = torch.empty((r, n), dtype=torch.float64, device=a.device)
+        jobs[j] = tn_svd_job(a.data_ptr(), m, n, U.data_ptr(), S.data_ptr(), Vt.data_ptr())
+        outs.append((U, S, Vt))
+        keep_alive.append(a)
+    if nj == 0:
+        return [], [], [], np.zeros(0, dtype=int)
+    ws = L.workspace(lib.tn_batched_svd_workspace(jobs, nj))
+    sweeps = (ctypes.c_int * nj)()
+    L.check(lib.tn_batched_svd(jobs, nj, int(max_sweeps), sweeps, L.ptr(ws), ws.numel(),
+                               L.stream_ptr()))
+    return ([o[0] for o in outs], [o[1] for o in outs], [o[2] for o in outs],
+            np.array(sweeps[:]))
 
 
 def config3_jobs(njobs, seed, chi_lo=16, chi_hi=512, d=2):

@@ -316,12 +316,33 @@ __global__ void __launch_bounds__(1024) bsort_rank(const double* __restrict__ si
 }
 
 struct OJob {
-  float* U;    // m x k row-major (FP32 output)
-  float* Vt;   // k x n row-major
+  void* U;     // m x k row-major (FP32 or FP64 output)
+  void* Vt;    // k x n row-major
   double* S;   // k
   int32_t m, n;
   int32_t tall;
+  int32_t f64;  // outputs are FP64
 };
+
+// FP64 input path: Wv vectors of job j from its row-major m x n matrix A
+// (columns when m >= n, rows otherwise).
+struct AJob {
+  const double* A;
+};
+
+__global__ void bgather_f64(double* __restrict__ Wv, const BJob* __restrict__ jobs,
+                            const AJob* __restrict__ ajobs, const OJob* __restrict__ outs) {
+  const int j = blockIdx.y;
+  const BJob J = jobs[j];
+  const OJob O = outs[j];
+  const double* A = ajobs[j].A;
+  const int64_t tot = (int64_t)J.L * J.k;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = e / J.L, i = e - v * J.L;
+    Wv[J.wv + e] = O.tall ? A[i * O.n + v] : A[v * O.n + i];
+  }
+}
 
 __global__ void bout(const double* __restrict__ Wv, const double* __restrict__ Zb,
                      const BJob* __restrict__ jobs, const OJob* __restrict__ outs,
@@ -341,14 +362,16 @@ __global__ void bout(const double* __restrict__ Wv, const double* __restrict__ Z
       int64_t src = pm[c];
       double v = O.tall ? (S[c] > 0.0 ? Wv[J.wv + src * J.L + i] / S[c] : 0.0)
                         : Zb[J.z + src * k + i];
-      O.U[idx] = (float)v;
+      if (O.f64) ((double*)O.U)[idx] = v;
+      else ((float*)O.U)[idx] = (float)v;
     } else {
       int64_t e = idx - (int64_t)O.m * k;
       int64_t r = e / O.n, c = e - r * O.n;
       int64_t src = pm[r];
       double v = O.tall ? Zb[J.z + src * k + c]
                         : (S[r] > 0.0 ? Wv[J.wv + src * J.L + c] / S[r] : 0.0);
-      O.Vt[e] = (float)v;
+      if (O.f64) ((double*)O.Vt)[e] = v;
+      else ((float*)O.Vt)[e] = (float)v;
     }
   }
   if (blockIdx.x == 0)
@@ -404,17 +427,13 @@ static double time_grouped(int FR, const std::vector<int>& ids, const std::vecto
   return rc == TN_OK ? ms : 1e30;
 }
 
-}  // namespace tn
 
-using namespace tn;
-
-extern "C" {
-
-size_t tn_batched_truncate_workspace(const tn_trunc_job* jobs, int njobs) {
+static size_t batched_ws_bytes(const std::vector<int64_t>& ms, const std::vector<int64_t>& ns) {
+  const int njobs = (int)ms.size();
   if (njobs <= 0) return 256;
   size_t wv = 0, z = 0, gi = 0, ei = 0, ai = 0, sv = 0;
   for (int j = 0; j < njobs; ++j) {
-    int64_t k = std::min(jobs[j].m, jobs[j].n), L = std::max(jobs[j].m, jobs[j].n);
+    int64_t k = std::min(ms[j], ns[j]), L = std::max(ms[j], ns[j]);
     int nblk = (int)ceil_div(std::max<int64_t>(k, 1), BJB);
     nblk += nblk % 2;
     int np = nblk / 2;
@@ -437,6 +456,7 @@ size_t tn_batched_truncate_workspace(const tn_trunc_job* jobs, int njobs) {
   a.take<BJob>((size_t)njobs);
   a.take<OJob>((size_t)njobs);
   a.take<GJob>((size_t)njobs);
+  a.take<AJob>((size_t)njobs);
   a.take<int>((size_t)njobs * 2);  // tile base + ids
   a.take<int>((size_t)njobs * 4);  // done, sweeps, kept, active
   a.take<unsigned long long>((size_t)njobs);
@@ -446,13 +466,20 @@ size_t tn_batched_truncate_workspace(const tn_trunc_job* jobs, int njobs) {
   return a.used + 4096;
 }
 
-int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshold, int chi_max,
-                        int max_sweeps, int* kept_out, int* sweeps_out, void* ws, size_t ws_bytes,
-                        void* stream) {
-  TN_REQUIRE(njobs >= 0 && jobs != nullptr, "tn_batched_truncate: bad job list");
+// One driver for both entry points: tj != nullptr forms each job's matrix as
+// the FP32 product L R (grouped GEMM); sj != nullptr gathers FP64 matrices.
+static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njobs,
+                          double rel_threshold, int chi_max, int max_sweeps, int* kept_out,
+                          int* sweeps_out, void* ws, size_t ws_bytes, void* stream) {
+  const char* who = tj ? "tn_batched_truncate" : "tn_batched_svd";
+  TN_REQUIRE(njobs >= 0 && (tj != nullptr || sj != nullptr), "%s: bad job list", who);
   if (njobs == 0) return TN_OK;
-  TN_REQUIRE(ws && ws_bytes >= tn_batched_truncate_workspace(jobs, njobs),
-             "tn_batched_truncate: workspace too small");
+  std::vector<int64_t> ms(njobs), ns(njobs);
+  for (int j = 0; j < njobs; ++j) {
+    ms[j] = tj ? tj[j].m : sj[j].m;
+    ns[j] = tj ? tj[j].n : sj[j].n;
+  }
+  TN_REQUIRE(ws && ws_bytes >= batched_ws_bytes(ms, ns), "%s: workspace too small", who);
   if (max_sweeps <= 0) max_sweeps = 40;
   hipStream_t st = as_stream(stream);
   // ---- host tables
@@ -463,12 +490,26 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
   std::vector<int64_t> sbase(njobs);
   int64_t wv = 0, z = 0, sv = 0;
   int maxsteps = 1, maxk = 1;
+  std::vector<AJob> aj(njobs);
   for (int j = 0; j < njobs; ++j) {
-    const tn_trunc_job& J = jobs[j];
-    TN_REQUIRE(J.m > 0 && J.n > 0 && J.k > 0 && J.L && J.R && J.U && J.S && J.Vt,
-               "tn_batched_truncate: job %d is invalid", j);
-    TN_REQUIRE(J.m <= (1 << 20) && J.n <= (1 << 20) && J.k <= (1 << 20),
-               "tn_batched_truncate: job %d too large", j);
+    tn_trunc_job J{};
+    if (tj) {
+      J = tj[j];
+      TN_REQUIRE(J.m > 0 && J.n > 0 && J.k > 0 && J.L && J.R && J.U && J.S && J.Vt,
+                 "%s: job %d is invalid", who, j);
+    } else {
+      J.m = sj[j].m;
+      J.n = sj[j].n;
+      J.k = 1;
+      J.U = sj[j].U;
+      J.S = sj[j].S;
+      J.Vt = sj[j].Vt;
+      TN_REQUIRE(J.m > 0 && J.n > 0 && sj[j].A && J.U && J.S && J.Vt, "%s: job %d is invalid",
+                 who, j);
+      aj[j].A = (const double*)sj[j].A;
+    }
+    TN_REQUIRE(J.m <= (1 << 20) && J.n <= (1 << 20) && J.k <= (1 << 20), "%s: job %d too large",
+               who, j);
     const int64_t k = std::min(J.m, J.n), L = std::max(J.m, J.n);
     BJob b{};
     b.wv = wv;
@@ -481,11 +522,13 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
     b.nchz = (int)ceil_div(k, BCH);
     b.gram_base = (int)g_job.size();
     b.eig_base = (int)e_job.size();
-    // The jobs' products are formed from FP32 factors with FP32 accumulation
-    // (~1e-7 relative rounding): orthogonality beyond 1e-10 is below the
-    // data's precision, and orthogonalising its noise-level vectors to
-    // L * eps costs up to three times the sweeps.
-    b.tol = std::max(1e-10, (double)L * 2.220446049250313e-16);
+    // The FP32 jobs' products are formed from FP32 factors with FP32
+    // accumulation (~1e-7 relative rounding): orthogonality beyond 1e-10 is
+    // below the data's precision, and orthogonalising its noise-level vectors
+    // to L * eps costs up to three times the sweeps.  FP64 matrices are
+    // orthogonalised to L * eps.
+    b.tol = tj ? std::max(1e-10, (double)L * 2.220446049250313e-16)
+               : (double)L * 2.220446049250313e-16;
     for (int p = 0; p < b.nblk / 2; ++p) {
       for (int c = 0; c < b.nch; ++c) { g_job.push_back(j); g_pair.push_back(p); g_chunk.push_back(c); }
       e_job.push_back(j);
@@ -497,7 +540,7 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
     const bool tall = J.m >= J.n;
     gj[j] = GJob{(const float*)J.L, (const float*)J.R, nullptr, (int)J.m, (int)J.n, (int)J.k,
                  tall ? 1 : (int64_t)J.n, tall ? (int64_t)J.m : 1};
-    oj[j] = OJob{(float*)J.U, (float*)J.Vt, J.S, (int)J.m, (int)J.n, tall ? 1 : 0};
+    oj[j] = OJob{J.U, J.Vt, J.S, (int)J.m, (int)J.n, tall ? 1 : 0, tj ? 0 : 1};
     wv += L * k;
     z += k * k;
     sv += k;
@@ -518,6 +561,7 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
   BJob* dbj = a.take<BJob>((size_t)njobs);
   OJob* doj = a.take<OJob>((size_t)njobs);
   GJob* dgj = a.take<GJob>((size_t)njobs);
+  AJob* daj = a.take<AJob>((size_t)njobs);
   int* dtile = a.take<int>((size_t)njobs);
   int* dids = a.take<int>((size_t)njobs);
   int* ddone = a.take<int>((size_t)njobs);
@@ -529,7 +573,7 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
   double* dS = a.take<double>((size_t)sv);
   int* dperm = a.take<int>((size_t)sv);
   int64_t* dsbase = a.take<int64_t>((size_t)njobs);
-  TN_REQUIRE(dsbase != nullptr, "tn_batched_truncate: workspace too small");
+  TN_REQUIRE(dsbase != nullptr, "%s: workspace too small", who);
   for (int j = 0; j < njobs; ++j) gj[j].C = dWv + bj[j].wv;
   std::vector<int> gt(ngi * 3), et(nei * 2), at(nai * 3);
   for (int i = 0; i < ngi; ++i) { gt[i] = g_job[i]; gt[ngi + i] = g_pair[i]; gt[2 * ngi + i] = g_chunk[i]; }
@@ -541,6 +585,7 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
   TN_HIP(hipMemcpyAsync(dbj, bj.data(), sizeof(BJob) * njobs, hipMemcpyHostToDevice, st));
   TN_HIP(hipMemcpyAsync(doj, oj.data(), sizeof(OJob) * njobs, hipMemcpyHostToDevice, st));
   TN_HIP(hipMemcpyAsync(dgj, gj.data(), sizeof(GJob) * njobs, hipMemcpyHostToDevice, st));
+  TN_HIP(hipMemcpyAsync(daj, aj.data(), sizeof(AJob) * njobs, hipMemcpyHostToDevice, st));
   TN_HIP(hipMemcpyAsync(dsbase, sbase.data(), sizeof(int64_t) * njobs, hipMemcpyHostToDevice, st));
   TN_HIP(hipMemsetAsync(ddone, 0, sizeof(int) * njobs, st));
   TN_HIP(hipMemsetAsync(dsweeps, 0, sizeof(int) * njobs, st));
@@ -548,8 +593,15 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
   int h_active = njobs;
   TN_HIP(hipMemcpyAsync(dactive, &h_active, sizeof(int), hipMemcpyHostToDevice, st));
 
-  // ---- 1. grouped contraction, tile shape per size bucket (autotuned once)
-  {
+  // ---- 1. the jobs' matrices into Wv: FP64 gather, or the grouped
+  //         contraction with the tile shape per size bucket (autotuned once)
+  if (sj) {
+    int64_t maxe = 1;
+    for (int j = 0; j < njobs; ++j) maxe = std::max<int64_t>(maxe, (int64_t)bj[j].L * bj[j].k);
+    hipLaunchKernelGGL(bgather_f64, dim3((unsigned)std::min<int64_t>(ceil_div(maxe, 256), 4096), njobs),
+                       dim3(256), 0, st, dWv, dbj, daj, doj);
+    TN_CHECK_LAUNCH();
+  } else {
     std::vector<int> byb[4];
     for (int j = 0; j < njobs; ++j) byb[bucket_of(gj[j].M, gj[j].N)].push_back(j);
     std::vector<int> use2, use4;
@@ -629,10 +681,41 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
   TN_HIP(hipStreamSynchronize(st));
   for (int j = 0; j < njobs; ++j)
     if (done[j] != 1) {
-      set_error(TN_ERR_NOCONV, "tn_batched_truncate: job %d did not converge", j);
+      set_error(TN_ERR_NOCONV, "%s: job %d did not converge", who, j);
       return TN_ERR_NOCONV;
     }
   return TN_OK;
+}
+
+}  // namespace tn
+
+extern "C" {
+
+size_t tn_batched_truncate_workspace(const tn_trunc_job* jobs, int njobs) {
+  std::vector<int64_t> ms, ns;
+  for (int j = 0; j < njobs && jobs; ++j) { ms.push_back(jobs[j].m); ns.push_back(jobs[j].n); }
+  return tn::batched_ws_bytes(ms, ns);
+}
+
+int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshold, int chi_max,
+                        int max_sweeps, int* kept_out, int* sweeps_out, void* ws, size_t ws_bytes,
+                        void* stream) {
+  TN_REQUIRE(jobs != nullptr || njobs == 0, "tn_batched_truncate: bad job list");
+  return tn::batched_driver(jobs, nullptr, njobs, rel_threshold, chi_max, max_sweeps, kept_out,
+                            sweeps_out, ws, ws_bytes, stream);
+}
+
+size_t tn_batched_svd_workspace(const tn_svd_job* jobs, int njobs) {
+  std::vector<int64_t> ms, ns;
+  for (int j = 0; j < njobs && jobs; ++j) { ms.push_back(jobs[j].m); ns.push_back(jobs[j].n); }
+  return tn::batched_ws_bytes(ms, ns);
+}
+
+int tn_batched_svd(const tn_svd_job* jobs, int njobs, int max_sweeps, int* sweeps_out, void* ws,
+                   size_t ws_bytes, void* stream) {
+  TN_REQUIRE(jobs != nullptr || njobs == 0, "tn_batched_svd: bad job list");
+  return tn::batched_driver(nullptr, jobs, njobs, 0.0, 0, max_sweeps, nullptr, sweeps_out, ws,
+                            ws_bytes, stream);
 }
 
 }  // extern "C"

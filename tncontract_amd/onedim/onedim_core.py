@@ -459,6 +459,54 @@ class MatrixProductStateCanonical(OneDimensionalTensorNetwork):
             self[start + 2] = S
             self[start + 3] = V[self.right_label, ] * S2_inv[self.left_label, ]
 
+    def apply_gate_layer(self, gate, firstsites, gate_outputs=None, gate_inputs=None, chi=None,
+                         threshold=1e-15):
+        """Extension (no counterpart in the reference): ``apply_gate(g, k,
+        gate_outputs, gate_inputs, chi, threshold)`` for every two-site gate g
+        (one Tensor for all bonds, or one per bond) and first site k in
+        `firstsites`, on pairwise disjoint bonds (|k - k'| >= 2), with the
+        bonds' SVDs in one batched launch.  Disjoint updates touch disjoint
+        Gamma/Lambda positions and read only Lambdas no other update writes,
+        so the result is that of the sequential calls."""
+        from tncontract_amd.batched import batched_svd
+        ks = list(firstsites)
+        gates = list(gate) if isinstance(gate, (list, tuple)) else [gate] * len(ks)
+        if len(gates) != len(ks):
+            raise ValueError("one gate per bond, or one gate for all bonds")
+        srt = sorted(ks)
+        if any(b - a < 2 for a, b in zip(srt, srt[1:])):
+            raise ValueError("apply_gate_layer: bonds must be disjoint (first sites >= 2 apart)")
+        blocks = []
+        for g, k in zip(gates, ks):
+            go, gi = self._gate_labels(g, gate_outputs, gate_inputs)
+            if len(gi) != 2:
+                raise NotImplementedError("apply_gate_layer applies two-site gates")
+            start = self.physical_site(k) - 1
+            end = self.physical_site(k + 1) + 1
+            t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
+            t = tsr.contract(t, g, self.phys_label, gi)
+            v, nrow = tsr._row_view(t, [go[0], self.left_label], move_all=False)
+            shape = tuple(v.shape)
+            m, n = int(np.prod(shape[:nrow])), int(np.prod(shape[nrow:]))
+            blocks.append((k, go, list(v.labels), shape, nrow, L.materialize(v.data).view(m, n)))
+        U, S, Vt, _ = batched_svd([b[5] for b in blocks])
+        for (k, go, labels, shape, nrow, _), u, s, vt in zip(blocks, U, S, Vt):
+            sv = s.cpu().numpy()
+            kk = int(np.count_nonzero((sv[:chi] if chi else sv) > threshold))
+            Ut = tsr.Tensor._wrap(L.contiguous(u[:, :kk]).view(shape[:nrow] + (kk,)),
+                                  labels[:nrow] + ["svd_in"])
+            Vm = tsr.Tensor._wrap(vt[:kk].view((kk,) + shape[nrow:]), ["svd_out"] + labels[nrow:])
+            Sm = tsr.Tensor._wrap(L.diag_embed(_upload(sv[:kk]), u.dtype),
+                                  [self.left_label, self.right_label])
+            Ut.replace_label(["svd_in", go[0]], [self.right_label, self.phys_label])
+            Vm.replace_label(["svd_out", go[1]], [self.left_label, self.phys_label])
+            start = self.physical_site(k) - 1
+            S1_inv = self._inverse(start)
+            S2_inv = self._inverse(start + 4)
+            self[start + 1] = S1_inv[self.right_label, ] * Ut[self.left_label, ]
+            self[start + 2] = Sm
+            self[start + 3] = Vm[self.right_label, ] * S2_inv[self.left_label, ]
+
     def swap_gate(self, i, chi=None, threshold=1e-15):
         """onedim_core.py:1188-1232: swap the physical indices of sites i and
         i+1 by re-splitting the two-site block with the first site's
