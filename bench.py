@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x3"])
     p.add_argument("--plain-steps", type=int, default=1)
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--eig-tol", type=float, default=None,
+                   help="Jacobi tolerance of the fast path's eigensolver (default: the "
+                        "structured sweep's own, 1e-10 for mixed precision)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     p.add_argument("--traffic", type=float, default=None,
                    help="HBM bytes per launch of the dominant kernel, from the calibrated "
@@ -576,7 +579,8 @@ def main():
     torch.cuda.synchronize()
 
     def fast():
-        return apply_mpo_compress(psi, mpo, chi=chi, threshold=1e-15, precision=args.precision)
+        return apply_mpo_compress(psi, mpo, chi=chi, threshold=1e-15, precision=args.precision,
+                                  eig_tol=args.eig_tol)
 
     def plain():
         out = tn.onedim.contract_mps_mpo(psi, mpo)
@@ -600,7 +604,8 @@ def main():
     L.prof_collect("gemm_f64", reset=True)
     L.prof_enable(True)
     st = SweepStats()
-    apply_mpo_compress(psi, mpo, chi=chi, threshold=1e-15, precision=args.precision, stats=st)
+    apply_mpo_compress(psi, mpo, chi=chi, threshold=1e-15, precision=args.precision, stats=st,
+                       eig_tol=args.eig_tol)
     torch.cuda.synchronize()
     L.prof_enable(False)
     prof = {tag: L.prof_collect(tag) for tag in L.PROF_TAGS}

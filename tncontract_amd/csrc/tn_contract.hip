@@ -456,12 +456,15 @@ __global__ void __launch_bounds__(256) splitk_reduce(const T* __restrict__ part,
 }
 
 // Split-K factor: with fewer output tiles than two per CU, split K so that
-// about 512 workgroups (two resident per CU) run, keeping >= 256 of K each.
+// about 512 workgroups (two resident per CU) run, keeping >= 64 of K (four
+// k-tiles) each.  A single FP64 tile runs at one CU's MFMA rate (~0.3
+// TFLOP/s), so few-tile, long-K products (V^T V, V^T A in the blocked QR)
+// are bound by how many CUs share their K.
 int splitk_factor(int64_t M, int64_t N, int64_t K) {
   const int64_t tiles = ceil_div(M, BM) * ceil_div(N, BN);
-  if (tiles >= 256 || K < 512) return 1;
+  if (tiles >= 256 || K < 128) return 1;
   const int64_t want = (512 + tiles / 2) / tiles;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(want, K / 256), 64));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(want, K / 64), 64));
 }
 
 int contract_launch(const void* A, const Modes& a_m, const Modes& a_k, const void* B,

@@ -316,31 +316,38 @@ __global__ void qr_build_v_kernel(const double* __restrict__ A, int64_t m, int64
   V[i] = (r == a) ? 1.0 : (r > a ? A[(j0 + r) + (j0 + a) * ld] : 0.0);
 }
 
-// T (nb x nb, row-major) from G = V^T V and tau (LAPACK dlarft, forward/columnwise).
-__global__ void __launch_bounds__(256) qr_larft_kernel(const double* __restrict__ G, int nb,
-                                                       const double* __restrict__ tau,
-                                                       double* __restrict__ T) {
-  __shared__ double Ts[QR_NB * QR_NB];
-  __shared__ double Gs[QR_NB * QR_NB];
+// T (nb x nb, row-major) from G = V^T V and tau (LAPACK dlarft,
+// forward/columnwise): T[b][b] = tau_b, T[a][b] = -tau_b sum_{a<=c<b}
+// T[a][c] G[c][b].  One wavefront, lane a owns row a of T in registers:
+// row a only needs its own earlier entries and G, so the 64 dependent
+// columns run without any barrier (fully unrolled, static register indices).
+__global__ void __launch_bounds__(64) qr_larft_kernel(const double* __restrict__ G, int nb,
+                                                      const double* __restrict__ tau,
+                                                      double* __restrict__ T) {
+  __shared__ double Gs[QR_NB][QR_NB + 1];
   __shared__ double ts[QR_NB];
-  for (int i = threadIdx.x; i < nb * nb; i += 256) {
-    Ts[i] = 0.0;
-    Gs[i] = G[i];
-  }
-  if (threadIdx.x < nb) ts[threadIdx.x] = tau[threadIdx.x];
+  const int a = threadIdx.x;
+  for (int i = a; i < nb * nb; i += 64) Gs[i / nb][i % nb] = G[i];
+  if (a < nb) ts[a] = tau[a];
   __syncthreads();
-  for (int b = 0; b < nb; ++b) {
-    const double tb = ts[b];
-    if (threadIdx.x < b) {
-      int a = threadIdx.x;
+  double Tr[QR_NB];
+#pragma unroll
+  for (int b = 0; b < QR_NB; ++b) {
+    double v = 0.0;
+    if (b < nb) {
       double acc = 0.0;
-      for (int c = a; c < b; ++c) acc += Ts[a * nb + c] * Gs[c * nb + b];
-      Ts[a * nb + b] = -tb * acc;
+#pragma unroll
+      for (int c = 0; c < b; ++c)
+        if (c >= a) acc = fma(Tr[c], Gs[c][b], acc);
+      v = a == b ? ts[b] : (a < b ? -ts[b] * acc : 0.0);
     }
-    if (threadIdx.x == 0) Ts[b * nb + b] = tb;
-    __syncthreads();
+    Tr[b] = v;
   }
-  for (int i = threadIdx.x; i < nb * nb; i += 256) T[i] = Ts[i];
+  if (a < nb) {
+#pragma unroll
+    for (int b = 0; b < QR_NB; ++b)
+      if (b < nb) T[a * nb + b] = Tr[b];
+  }
 }
 
 __global__ void qr_init_from_kernel(const double* __restrict__ X, double* __restrict__ Q, int64_t m,
@@ -585,7 +592,7 @@ int qr_factor(double* W, int64_t m, int64_t n, const QrWs& ws, hipStream_t st) {
                            mk({nbo}, {mr}), ws.G, mk({nbo}, {nbo}), mk({nbo}, {1}), TN_F64,
                            TN_PREC_NATIVE, 1.0, 0.0, ws.part, ws.part_bytes, st));
     double* T = ws.Tall + (j0 / QR_NB) * QR_NB * QR_NB;
-    hipLaunchKernelGGL(qr_larft_kernel, dim3(1), dim3(256), 0, st, ws.G, nbo, ws.tau + j0, T);
+    hipLaunchKernelGGL(qr_larft_kernel, dim3(1), dim3(64), 0, st, ws.G, nbo, ws.tau + j0, T);
     TN_CHECK_LAUNCH();
     const int64_t nt = n - (j0 + nbo);
     if (nt > 0) {
