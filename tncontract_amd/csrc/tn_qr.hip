@@ -23,29 +23,33 @@ constexpr int PT = 512;  // panel threads (2 waves per SIMD: 256 VGPRs per threa
 constexpr int NWAVE = PT / 64;
 
 // Block-wide sum of nv (<= NV) values; every thread receives the totals.
+// One barrier per call: each wave's partials go to one of two LDS buffers
+// (alternating with `par`) and every thread sums the NWAVE partials itself
+// in wave order.  A buffer is written again only two calls later, after the
+// intervening call's barrier, which every thread reaches after its reads.
 template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], int nv, double* lds, double* out) {
+__device__ __forceinline__ void block_sum(double (&v)[NV], int nv, double (*lds)[NWAVE * NV],
+                                          int& par) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* buf = lds[par];
+  par ^= 1;
 #pragma unroll
   for (int q = 0; q < NV; ++q)
     if (q < nv) v[q] = wave_sum(v[q]);
   if (lane == 0) {
 #pragma unroll
     for (int q = 0; q < NV; ++q)
-      if (q < nv) lds[wave * NV + q] = v[q];
-  }
-  __syncthreads();
-  if (threadIdx.x < nv) {
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < NWAVE; ++w) s += lds[w * NV + threadIdx.x];
-    out[threadIdx.x] = s;
+      if (q < nv) buf[wave * NV + q] = v[q];
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NV; ++q)
-    if (q < nv) v[q] = out[q];
-  __syncthreads();
+    if (q < nv) {
+      double acc = 0.0;
+#pragma unroll
+      for (int w = 0; w < NWAVE; ++w) acc += buf[w * NV + q];
+      v[q] = acc;
+    }
 }
 
 // Factor columns [c0, c0+w) (w <= W) over rows [c0, m) of A (column-major,
@@ -58,12 +62,11 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
                                                       int64_t ld, int64_t c0, int w,
                                                       double* __restrict__ tau,
                                                       double* __restrict__ Tp) {
-  __shared__ double lds[NWAVE * W];
-  __shared__ double out[W];
+  __shared__ double lds[2][NWAVE * W];
+  int par = 0;
   __shared__ double piv[W];       // pivot value of each column (one slot each)
   __shared__ double gsh[W][W];    // v_a . v_b, a < b
   __shared__ double taus[W];
-  __shared__ double Tsh[W][W];
   const int t = threadIdx.x;
   double x[R][W];
 #pragma unroll
@@ -99,7 +102,7 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
         for (int a = 0; a < j - 1; ++a) vals[1 + a] += x[0][a];
       }
       if (t == j) piv[j] = x[0][j];
-      block_sum<W>(vals, j >= 2 ? j : 1, lds, out);
+      block_sum<W>(vals, j >= 2 ? j : 1, lds, par);
       if (j >= 2 && t == 0) {
 #pragma unroll
         for (int a = 0; a < j - 1; ++a) gsh[a][j - 1] = vals[1 + a];
@@ -136,7 +139,7 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
 #pragma unroll
           for (int q = j + 1; q < W; ++q) d[q] += x[0][q];
         }
-        block_sum<W>(d, w, lds, out);
+        block_sum<W>(d, w, lds, par);
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           int64_t r = c0 + t + (int64_t)PT * i;
@@ -169,7 +172,7 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
 #pragma unroll
         for (int a = 0; a < b; ++a) vals[a] += x[0][a];
       }
-      block_sum<W>(vals, b, lds, out);
+      block_sum<W>(vals, b, lds, par);
       if (t == 0) {
 #pragma unroll
         for (int a = 0; a < b; ++a) gsh[a][b] = vals[a];
@@ -177,22 +180,28 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
     }
   }
   __syncthreads();
-  if (t == 0) {
-    // T (upper triangular, w x w): T[b][b] = tau_b, T[0:b, b] = -tau_b T[0:b,0:b] G[0:b, b]
-    for (int a = 0; a < W; ++a)
-      for (int b = 0; b < W; ++b) Tsh[a][b] = 0.0;
-    for (int b = 0; b < w; ++b) {
-      Tsh[b][b] = taus[b];
-      for (int a = 0; a < b; ++a) {
+  if (t < w) {
+    // T (upper triangular, w x w): T[b][b] = tau_b, T[a][b] = -tau_b
+    // sum_{a<=c<b} T[a][c] G[c][b]; thread a builds row a in registers (a row
+    // needs only its own earlier entries), so the rows run in parallel.
+    const int a = t;
+    double Tr[W];
+#pragma unroll
+    for (int b = 0; b < W; ++b) {
+      double v = 0.0;
+      if (b < w) {
         double acc = 0.0;
-        for (int c = a; c < b; ++c) acc += Tsh[a][c] * gsh[c][b];
-        Tsh[a][b] = -taus[b] * acc;
+#pragma unroll
+        for (int c = 0; c < b; ++c)
+          if (c >= a) acc += Tr[c] * gsh[c][b];
+        v = a == b ? taus[b] : (a < b ? -taus[b] * acc : 0.0);
       }
+      Tr[b] = v;
     }
-    for (int a = 0; a < w; ++a) {
-      tau[c0 + a] = taus[a];
-      for (int b = 0; b < w; ++b) Tp[a * w + b] = Tsh[a][b];
-    }
+    tau[c0 + a] = taus[a];
+#pragma unroll
+    for (int b = 0; b < W; ++b)
+      if (b < w) Tp[a * w + b] = Tr[b];
   }
 #pragma unroll
   for (int i = 0; i < R; ++i) {
