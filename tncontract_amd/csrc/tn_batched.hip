@@ -129,7 +129,8 @@ __global__ void __launch_bounds__(256) bjac_gram(double* __restrict__ Wv, const 
                                                  const int* __restrict__ it_pair,
                                                  const int* __restrict__ it_chunk,
                                                  const int* __restrict__ done, int g,
-                                                 double* __restrict__ Gpart) {
+                                                 double* __restrict__ Gpart,
+                                                 unsigned long long* __restrict__ work) {
   __shared__ double Ps[64][BJ2 + 1];
   const int item = blockIdx.x;
   const int j = it_job[item];
@@ -141,6 +142,7 @@ __global__ void __launch_bounds__(256) bjac_gram(double* __restrict__ Wv, const 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int ti = wave >> 1, tj = wave & 1;
   const int64_t r0 = (int64_t)ch * BCH, r1 = min((int64_t)J.L, r0 + BCH);
+  if (work && tid == 0) atomicAdd(work, (unsigned long long)(r1 - r0));
   const double* base = Wv + J.wv;
   f64x4b acc = {0, 0, 0, 0};
   for (int64_t rb = r0; rb < r1; rb += 64) {
@@ -175,7 +177,8 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
                                                 const int* __restrict__ e_pair,
                                                 const int* __restrict__ done,
                                                 double* __restrict__ Qall, int* __restrict__ skip,
-                                                unsigned long long* __restrict__ offmax) {
+                                                unsigned long long* __restrict__ offmax,
+                                                unsigned long long* __restrict__ work) {
   __shared__ EigLds sm;
   const int item = blockIdx.x, tid = threadIdx.x;
   const int j = e_job[item];
@@ -197,6 +200,7 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
     return;
   }
   if (tid == 0) skip[item] = 0;
+  if (work && tid == 0) atomicAdd(work + 1, 1ull);
   const int cur = eig_rotate(sm, J.tol);
   double* out = Qall + (int64_t)item * (BJ2 * BJ2);
   for (int e = tid; e < BJ2 * BJ2; e += 256) out[e] = sm.Q[cur][e / BJ2][e % BJ2];
@@ -210,7 +214,8 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
                                                   const int* __restrict__ a_chunk,
                                                   const int* __restrict__ done, int g,
                                                   const double* __restrict__ Qall,
-                                                  const int* __restrict__ skip) {
+                                                  const int* __restrict__ skip,
+                                                  unsigned long long* __restrict__ work) {
   __shared__ double Q[BJ2][BJ2 + 1];
   __shared__ ApplyTile Tw;
   const int item = blockIdx.x;
@@ -223,6 +228,10 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
   const int s = g % (J.nblk - 1);
   const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
   const double* Qg = Qall + (int64_t)eidx * BJ2 * BJ2;
+  if (work && threadIdx.x == 0) {
+    const int64_t r = (int64_t)(ch < J.nch ? ch : ch - J.nch) * BCH;
+    atomicAdd(work + 2, (unsigned long long)(min((int64_t)(ch < J.nch ? J.L : J.k), r + BCH) - r));
+  }
   if (ch < J.nch) {
     const int64_t r = (int64_t)ch * BCH;
     apply_mfma(Wv + J.wv, J.L, J.k, r, min((int64_t)J.L, r + BCH), I, K2, Qg, Q, Tw);
@@ -460,6 +469,7 @@ static size_t batched_ws_bytes(const std::vector<int64_t>& ms, const std::vector
   a.take<int>((size_t)njobs * 2);  // tile base + ids
   a.take<int>((size_t)njobs * 4);  // done, sweeps, kept, active
   a.take<unsigned long long>((size_t)njobs);
+  a.take<unsigned long long>(3);  // work counters
   a.take<double>(sv * 2);
   a.take<int>(sv);
   a.take<int64_t>((size_t)njobs);
@@ -569,6 +579,7 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   int* dkept = a.take<int>((size_t)njobs);
   int* dactive = a.take<int>((size_t)njobs);
   unsigned long long* doff = a.take<unsigned long long>((size_t)njobs);
+  unsigned long long* dwork = a.take<unsigned long long>(3);
   double* dsig = a.take<double>((size_t)sv);
   double* dS = a.take<double>((size_t)sv);
   int* dperm = a.take<int>((size_t)sv);
@@ -590,6 +601,10 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   TN_HIP(hipMemsetAsync(ddone, 0, sizeof(int) * njobs, st));
   TN_HIP(hipMemsetAsync(dsweeps, 0, sizeof(int) * njobs, st));
   TN_HIP(hipMemsetAsync(doff, 0, sizeof(unsigned long long) * njobs, st));
+  // rows through the Gram and rotation kernels and eigensolves that rotated,
+  // counted only while launch timing is on (they give the Jacobi rooflines)
+  unsigned long long* cnt = prof_on() ? dwork : nullptr;
+  if (cnt) TN_HIP(hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), st));
   int h_active = njobs;
   TN_HIP(hipMemcpyAsync(dactive, &h_active, sizeof(int), hipMemcpyHostToDevice, st));
 
@@ -623,14 +638,26 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
       }
       (fr == 2 ? use2 : use4).insert((fr == 2 ? use2 : use4).end(), byb[bkt].begin(), byb[bkt].end());
     }
+    auto work = [&](const std::vector<int>& ids) {
+      double f = 0.0;
+      for (int j : ids) f += 2.0 * gj[j].M * (double)gj[j].N * gj[j].K;
+      return f;
+    };
+    auto bytes = [&](const std::vector<int>& ids) {
+      double b = 0.0;
+      for (int j : ids)
+        b += 4.0 * ((double)gj[j].M * gj[j].K + (double)gj[j].K * gj[j].N) +
+             8.0 * (double)gj[j].M * gj[j].N;
+      return b;
+    };
     {
-      ProfScope p(PROF_GEMM_F32, st, 0.0, 0.0);
+      ProfScope p(PROF_GEMM_F32, st, work(use2), bytes(use2));
       TN_TRY(launch_grouped(2, use2, gj, dgj, dtile, dids, st));
     }
     if (!use4.empty()) {
       // second grouped launch needs its own tables: the first launch's are in use
       TN_HIP(hipStreamSynchronize(st));
-      ProfScope p(PROF_GEMM_F32, st, 0.0, 0.0);
+      ProfScope p(PROF_GEMM_F32, st, work(use4), bytes(use4));
       TN_TRY(launch_grouped(4, use4, gj, dgj, dtile, dids, st));
     }
   }
@@ -641,20 +668,21 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   const int poll = std::max(8, maxsteps);
   int g = 0;
   for (; g < maxsteps * max_sweeps; ++g) {
+    // work is counted on the device (cnt) and added once at the end
     {
       ProfScope p(PROF_JAC_GRAM, st, 0.0, 0.0);
       hipLaunchKernelGGL(bjac_gram, dim3(ngi), dim3(256), 0, st, dWv, dbj, dgt, dgt + ngi,
-                         dgt + 2 * ngi, ddone, g, dG);
+                         dgt + 2 * ngi, ddone, g, dG, cnt);
     }
     {
       ProfScope p(PROF_JAC_EIG, st, 0.0, 0.0);
       hipLaunchKernelGGL(bjac_eig, dim3(nei), dim3(256), 0, st, dG, dbj, det, det + nei, ddone, dQ,
-                         dskip, doff);
+                         dskip, doff, cnt);
     }
     {
       ProfScope p(PROF_JAC_APPLY, st, 0.0, 0.0);
       hipLaunchKernelGGL(bjac_apply, dim3(nai), dim3(256), 0, st, dWv, dZ, dbj, dat, dat + nai,
-                         dat + 2 * nai, ddone, g, dQ, dskip);
+                         dat + 2 * nai, ddone, g, dQ, dskip, cnt);
     }
     hipLaunchKernelGGL(bjac_sweep_end, dim3((unsigned)ceil_div(njobs, 256)), dim3(256), 0, st, dbj,
                        njobs, g, max_sweeps, ddone, dsweeps, doff, dactive);
@@ -678,7 +706,17 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   TN_HIP(hipMemcpyAsync(done.data(), ddone, sizeof(int) * njobs, hipMemcpyDeviceToHost, st));
   if (sweeps_out)
     TN_HIP(hipMemcpyAsync(sweeps_out, dsweeps, sizeof(int) * njobs, hipMemcpyDeviceToHost, st));
+  unsigned long long hw[3] = {0, 0, 0};
+  if (cnt) TN_HIP(hipMemcpyAsync(hw, cnt, sizeof(hw), hipMemcpyDeviceToHost, st));
   TN_HIP(hipStreamSynchronize(st));
+  if (cnt) {
+    // a Gram row of a 32-column block pair: 32 x 32 multiply-adds, 32 values
+    // read; a rotated row: the same product, 32 values read and written; a
+    // 32 x 32 eigensolve: 31 rotation rounds of about 9216 flops
+    prof_add(PROF_JAC_GRAM, 2.0 * BJ2 * BJ2 * (double)hw[0], 8.0 * BJ2 * (double)hw[0]);
+    prof_add(PROF_JAC_EIG, 31.0 * 9216.0 * (double)hw[1], 16.0 * BJ2 * BJ2 * (double)hw[1]);
+    prof_add(PROF_JAC_APPLY, 2.0 * BJ2 * BJ2 * (double)hw[2], 16.0 * BJ2 * (double)hw[2]);
+  }
   for (int j = 0; j < njobs; ++j)
     if (done[j] != 1) {
       set_error(TN_ERR_NOCONV, "%s: job %d did not converge", who, j);

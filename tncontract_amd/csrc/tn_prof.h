@@ -22,6 +22,9 @@ bool prof_on();
 // Record a start event before a launch; returns an opaque slot (or -1).
 int prof_start(int tag, hipStream_t st, double flops, double bytes);
 void prof_stop(int slot, hipStream_t st);
+// Add work to a tag's totals without a launch of its own: for kernels whose
+// work is only known after they ran (counted on the device).
+void prof_add(int tag, double flops, double bytes);
 
 struct ProfScope {
   int slot;

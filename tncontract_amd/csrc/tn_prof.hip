@@ -17,6 +17,7 @@ std::mutex g_mu;
 bool g_on = false;
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_pool;
+double g_extra[PROF_NTAGS][2] = {};
 
 hipEvent_t get_event() {
   if (!g_pool.empty()) {
@@ -48,6 +49,13 @@ void prof_stop(int slot, hipStream_t st) {
   (void)hipEventRecord(g_recs[slot].b, st);
 }
 
+void prof_add(int tag, double flops, double bytes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_on || tag < 0 || tag >= PROF_NTAGS) return;
+  g_extra[tag][0] += flops;
+  g_extra[tag][1] += bytes;
+}
+
 }  // namespace tn
 
 using namespace tn;
@@ -74,6 +82,8 @@ extern "C" int tn_prof_collect(int tag, double* total_ms, int64_t* launches, dou
     by += r.bytes;
     ++n;
   }
+  fl += g_extra[tag][0];
+  by += g_extra[tag][1];
   if (total_ms) *total_ms = ms;
   if (launches) *launches = n;
   if (flops) *flops = fl;
@@ -85,6 +95,7 @@ extern "C" int tn_prof_collect(int tag, double* total_ms, int64_t* launches, dou
       g_pool.push_back(r.b);
     }
     g_recs.clear();
+    for (auto& e : g_extra) e[0] = e[1] = 0.0;
   }
   return TN_OK;
 }
