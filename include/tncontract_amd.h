@@ -135,6 +135,24 @@ size_t tn_qr_workspace(int64_t m, int64_t n, int dtype);
 int tn_qr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m, int64_t n,
           void* Q, void* R, int dtype, void* ws, size_t ws_bytes, void* stream);
 
+/* ---- Two-pass Cholesky QR (row space of a variational site update) -------
+ * A = Q R for the m x n FP64 matrix A, m >= n (decoded as in tn_qr): Q m x n
+ * row-major with orthonormal columns, R n x n row-major upper triangular
+ * with a positive diagonal.  Replaces tensor_lq (tensor.py:1058-1109) where
+ * variational_compress calls it (onedim_core.py:560-631): there only the row
+ * space matters (Q becomes the site, L = R^T is absorbed into the
+ * neighbour), so the factors may differ from Householder's by the sign of
+ * each row.  Method: G = A^T A = L L^T (blocked Cholesky), Q1 = A L^{-T},
+ * then the same on Q1; R = (L1 L2)^T.  *orth_out (host, may be NULL)
+ * receives max |Q^T Q - I|.  Returns TN_ERR_NOCONV (outputs unusable) when a
+ * first-pass pivot falls below 1e-14 of the largest diagonal of A^T A or the
+ * orthogonality exceeds orth_tol (<= 0 -> 1e-12): the caller then uses
+ * tn_qr. */
+size_t tn_cholqr_workspace(int64_t m, int64_t n);
+int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m, int64_t n,
+              double* Q, double* R, double orth_tol, double* orth_out, void* ws, size_t ws_bytes,
+              void* stream);
+
 /* ---- SVD (np.linalg.svd(full_matrices=False), tensor.py:915) ------------
  * Thin SVD of the m x n matrix A (decoded as in tn_qr): U m x k row-major,
  * S k (FP64, descending), Vt k x n row-major, k = min(m,n).  One-sided

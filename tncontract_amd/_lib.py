@@ -86,6 +86,9 @@ def lib():
         "tn_contract_ws": (i32, [vp, mp, mp, vp, mp, mp, vp, mp, mp, i32, i32, dbl, dbl, vp, sz, vp]),
         "tn_qr_workspace": (sz, [i64, i64, i32]),
         "tn_qr": (i32, [vp, mp, mp, i64, i64, vp, vp, i32, vp, sz, vp]),
+        "tn_cholqr_workspace": (sz, [i64, i64]),
+        "tn_cholqr": (i32, [vp, mp, mp, i64, i64, vp, vp, dbl, ctypes.POINTER(ctypes.c_double),
+                            vp, sz, vp]),
         "tn_prof_enable": (i32, [i32]),
         "tn_prof_collect": (i32, [i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), i32]),

@@ -1,0 +1,301 @@
+// Two-pass Cholesky QR of a tall FP64 matrix: A = Q R with orthonormal Q.
+//
+// The variational sweep (onedim_core.py:560-631) re-orthogonalises one site
+// per update through tensor_lq (tensor.py:1058-1109), whose only use there
+// is the row space: Q becomes the site, L is absorbed into the neighbour, so
+// any A = L Q with orthonormal Q gives the same state.  Householder QR of
+// these 1024 x 512 blocks is a chain of latency-bound panel kernels;
+// Cholesky QR is three GEMM-shaped products per pass plus one small
+// triangular factorisation per 64 columns:
+//   G = A^T A = L L^T,  Q1 = A L^{-T};  the same on Q1 (Q1 = Q2 L2^T);
+//   A = Q2 (L1 L2)^T.
+// The second pass restores orthogonality to rounding when cond(A) is well
+// below eps^{-1/2}; the driver checks ||Q^T Q - I||_max at the end and
+// reports ill-conditioned inputs (TN_ERR_NOCONV) so the caller can fall back
+// to Householder.
+#include "tn_internal.h"
+#include "tn_prof.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace tn {
+
+int splitk_factor(int64_t M, int64_t N, int64_t K);
+
+namespace {
+
+constexpr int QB = 64;  // Cholesky block
+
+// S[n + i, j] = A(i, j): the m x n matrix under the n x n Gram block of the
+// stacked (n + m) x n column-major buffer S (ld = n + m).
+__global__ void cq_gather(const double* __restrict__ A, Modes rows, Modes cols, int64_t m,
+                          int64_t n, double* __restrict__ S) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= m * n) return;
+  const int64_t j = idx / m, i = idx - j * m;
+  S[j * (n + m) + n + i] = A[mode_offset(rows, i) + mode_offset(cols, j)];
+}
+
+// pivmin = thr * max_i S[i, i] over the leading n x n block; one block of 256.
+__global__ void __launch_bounds__(256) cq_pivmin(const double* __restrict__ S, int64_t ld,
+                                                 int64_t n, double thr,
+                                                 double* __restrict__ pivmin) {
+  __shared__ double part[4];
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) v = fmax(v, S[i * ld + i]);
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) *pivmin = thr * fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+}
+
+// Factor the w x w diagonal block at (j0, j0) of S (column-major, ld):
+// L11 (lower, zeros above) into F at the same position and X = L11^{-1}
+// (QB x QB, column-major).  256 threads: thread (q, t) keeps row t of the
+// block, columns 16q..16q+15, in registers, for both the Cholesky
+// elimination and X, which is accumulated by the same column sweep
+// (X <- I; at step c: X[c,:] /= L[c,c], X[t,:] -= L[t,c] X[c,:] for t > c).
+// Two barriers per column.  A pivot not above *pivmin sets *fail and is
+// replaced by *pivmin (or 1) so that everything stays finite; the caller
+// discards the result.  Padding rows/columns (>= w) are zero with unit
+// pivots.
+__global__ void __launch_bounds__(256) cq_diag(const double* __restrict__ S, int64_t ld,
+                                               int64_t j0, int w,
+                                               const double* __restrict__ pivmin,
+                                               double* __restrict__ F, double* __restrict__ X,
+                                               int* __restrict__ fail) {
+  __shared__ double col[QB];
+  __shared__ double xrow[QB];
+  __shared__ double piv;
+  const int t = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const double pm = *pivmin;
+  bool bad = false;
+  double r[16], x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = 16 * q + k;
+    r[k] = (t < w && c < w) ? S[(j0 + c) * ld + j0 + t] : 0.0;
+    x[k] = c == t ? 1.0 : 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < QB; ++c) {
+    const int qc = c >> 4, kc = c & 15;
+    if (q == qc && t == c) piv = r[kc];
+    __syncthreads();
+    double d = piv;
+    if (c < w) {
+      if (!(d > pm)) {
+        bad = true;
+        d = pm > 0.0 ? pm : 1.0;
+      }
+    } else {
+      d = 1.0;
+    }
+    const double sq = sqrt(d);
+    const double inv = 1.0 / sq;
+    if (q == qc) {
+      if (t == c) r[kc] = sq;
+      if (t > c) {
+        r[kc] *= inv;
+        col[t] = r[kc];
+      }
+    }
+    if (t == c) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        x[k] *= inv;
+        xrow[16 * q + k] = x[k];
+      }
+    }
+    __syncthreads();
+    if (t > c) {
+      const double l = col[t];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if (16 * q + k > c) r[k] -= l * col[16 * q + k];
+        x[k] -= l * xrow[16 * q + k];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = 16 * q + k;
+    if (t < w && c < w) F[(j0 + c) * ld + j0 + t] = c <= t ? r[k] : 0.0;
+    X[c * QB + t] = (t < w && c < w) ? x[k] : 0.0;
+  }
+  if (bad && threadIdx.x == 0) fail[0] = 1;
+}
+
+// orth = max |G - I| over the n x n block G (column-major, ld); block c
+// scans column c; orth must start at zero (non-negative doubles order as
+// their bit patterns).
+__global__ void __launch_bounds__(256) cq_orth(const double* __restrict__ G, int64_t ld,
+                                               int64_t n, unsigned long long* __restrict__ orth) {
+  __shared__ double part[4];
+  const int64_t c = blockIdx.x;
+  double v = 0.0;
+  for (int64_t r = threadIdx.x; r < n; r += 256)
+    v = fmax(v, fabs(G[c * ld + r] - (r == c ? 1.0 : 0.0)));
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double b = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+    atomicMax(orth, (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+// Lo (n x n column-major) = lower triangle of the leading block of F (ld).
+__global__ void cq_lower(const double* __restrict__ F, int64_t ld, int64_t n,
+                         double* __restrict__ Lo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t c = idx / n, r = idx - c * n;
+  Lo[idx] = r >= c ? F[c * ld + r] : 0.0;
+}
+
+// Q (m x n row-major) = F[n:, :] (column-major, ld), through an LDS tile.
+__global__ void __launch_bounds__(256) cq_qout(const double* __restrict__ F, int64_t ld,
+                                               int64_t m, int64_t n, double* __restrict__ Q) {
+  __shared__ double tile[32][33];
+  const int64_t i0 = (int64_t)blockIdx.x * 32, j0 = (int64_t)blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty 0..7
+  for (int jj = ty; jj < 32; jj += 8) {
+    const int64_t i = i0 + tx, j = j0 + jj;
+    tile[jj][tx] = (i < m && j < n) ? F[j * ld + n + i] : 0.0;
+  }
+  __syncthreads();
+  for (int ii = ty; ii < 32; ii += 8) {
+    const int64_t i = i0 + ii, j = j0 + tx;
+    if (i < m && j < n) Q[i * n + j] = tile[tx][ii];
+  }
+}
+
+struct CqWs {
+  double *B0, *B1, *L1, *X, *pivmin, *part;
+  unsigned long long* orth;
+  int* fail;
+  size_t part_bytes;
+};
+
+size_t cq_layout(int64_t m, int64_t n, char* base, size_t cap, CqWs* ws) {
+  Arena a{base, cap, 0};
+  CqWs w{};
+  w.B0 = a.take<double>((size_t)((n + m) * n));
+  w.B1 = a.take<double>((size_t)((n + m) * n));
+  w.L1 = a.take<double>((size_t)(n * n));
+  w.X = a.take<double>((size_t)QB * QB);
+  w.pivmin = a.take<double>(2);
+  w.orth = a.take<unsigned long long>(1);
+  w.fail = a.take<int>(1);
+  const int S = std::max(splitk_factor(n, n, m), splitk_factor(n, n, n));
+  w.part_bytes = S > 1 ? (size_t)S * n * n * sizeof(double) : 0;
+  w.part = S > 1 ? a.take<double>((size_t)S * n * n) : nullptr;
+  if (ws) *ws = w;
+  return a.used;
+}
+
+// One Cholesky QR pass.  S (ld = n + m): W = S[n:, :] holds the m x n input;
+// its Gram W^T W is formed into S[:n, :] and the stacked matrix is
+// eliminated block by block: F[:n, :] receives L (lower, W^T W = L L^T) and
+// F[n:, :] receives Q = W L^{-T}, because the rows of W go through the same
+// column eliminations as the rows of the Gram.  S is destroyed; F is
+// cleared first.
+int cq_pass(double* S, double* F, int64_t m, int64_t n, double thr, const CqWs& ws,
+            hipStream_t st) {
+  const int64_t ld = n + m;
+  TN_TRY(contract_launch(S + n, mk({n}, {ld}), mk({m}, {1}), S + n, mk({m}, {1}), mk({n}, {ld}),
+                         S, mk({n}, {1}), mk({n}, {ld}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, ws.part,
+                         ws.part_bytes, st));
+  hipLaunchKernelGGL(cq_pivmin, dim3(1), dim3(256), 0, st, S, ld, n, thr, ws.pivmin);
+  TN_HIP(hipMemsetAsync(F, 0, sizeof(double) * (size_t)(ld * n), st));
+  for (int64_t j0 = 0; j0 < n; j0 += QB) {
+    const int w = (int)std::min<int64_t>(QB, n - j0);
+    hipLaunchKernelGGL(cq_diag, dim3(1), dim3(256), 0, st, S, ld, j0, w, ws.pivmin, F, ws.X,
+                       ws.fail);
+    TN_CHECK_LAUNCH();
+    // [L21; Q_j] = [G21; W_j] L11^{-T}:  B[k][c] = X[c][k] = X[k * QB + c]
+    const int64_t rows = ld - j0 - w;
+    TN_TRY(contract_launch(S + j0 * ld + j0 + w, mk({rows}, {1}), mk({w}, {ld}), ws.X,
+                           mk({w}, {QB}), mk({w}, {1}), F + j0 * ld + j0 + w, mk({rows}, {1}),
+                           mk({w}, {ld}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, nullptr, 0, st));
+    const int64_t m2 = n - j0 - w;
+    if (m2 <= 0) break;
+    // [G22; W_rest] -= [L21; Q_j] L21^T
+    const double* P = F + j0 * ld + j0 + w;
+    TN_TRY(contract_launch(P, mk({rows}, {1}), mk({w}, {ld}), P, mk({w}, {ld}), mk({m2}, {1}),
+                           S + (j0 + w) * ld + j0 + w, mk({rows}, {1}), mk({m2}, {ld}), TN_F64,
+                           TN_PREC_NATIVE, -1.0, 1.0, nullptr, 0, st));
+  }
+  return TN_OK;
+}
+
+}  // namespace
+}  // namespace tn
+
+using namespace tn;
+
+extern "C" size_t tn_cholqr_workspace(int64_t m, int64_t n) {
+  if (m <= 0 || n <= 0) return 256;
+  return cq_layout(m, n, nullptr, 0, nullptr) + 256;
+}
+
+extern "C" int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m,
+                         int64_t n, double* Q, double* R, double orth_tol, double* orth_out,
+                         void* ws, size_t ws_bytes, void* stream) {
+  Modes rows = to_modes(a_rows), cols = to_modes(a_cols);
+  TN_REQUIRE(m >= 0 && n >= 0 && m >= n, "tn_cholqr: needs m >= n >= 0 (got %lld x %lld)",
+             (long long)m, (long long)n);
+  TN_REQUIRE(modes_size(rows) == m && modes_size(cols) == n, "tn_cholqr: modes do not match m x n");
+  if (orth_out) *orth_out = 0.0;
+  if (n == 0) return TN_OK;
+  TN_REQUIRE(A && Q && R, "tn_cholqr: null pointer");
+  TN_REQUIRE(ws && ws_bytes >= tn_cholqr_workspace(m, n), "tn_cholqr: workspace too small");
+  if (orth_tol <= 0.0) orth_tol = 1e-12;
+  hipStream_t st = as_stream(stream);
+  CqWs w{};
+  cq_layout(m, n, (char*)ws, ws_bytes, &w);
+  const int64_t ld = n + m;
+  TN_HIP(hipMemsetAsync(w.fail, 0, sizeof(int), st));
+  TN_HIP(hipMemsetAsync(w.orth, 0, sizeof(unsigned long long), st));
+  hipLaunchKernelGGL(cq_gather, dim3((unsigned)ceil_div(m * n, 256)), dim3(256), 0, st,
+                     (const double*)A, rows, cols, m, n, w.B0);
+  TN_CHECK_LAUNCH();
+  // pass 1 (B0 -> B1): pivots must stay above 1e-14 of the largest diagonal
+  // of A^T A (cond(A) up to ~1e7)
+  TN_TRY(cq_pass(w.B0, w.B1, m, n, 1e-14, w, st));
+  const unsigned nb = (unsigned)ceil_div(n * n, 256);
+  hipLaunchKernelGGL(cq_lower, dim3(nb), dim3(256), 0, st, w.B1, ld, n, w.L1);
+  // pass 2 (B1 -> B0) on Q1, nearly orthonormal: its Gram's pivots stay near 1
+  TN_TRY(cq_pass(w.B1, w.B0, m, n, 1e-2, w, st));
+  hipLaunchKernelGGL(cq_qout, dim3((unsigned)ceil_div(m, 32), (unsigned)ceil_div(n, 32)),
+                     dim3(256), 0, st, w.B0, ld, m, n, Q);
+  TN_CHECK_LAUNCH();
+  // R = (L1 L2)^T: the column-major product L1 L2 read row-major
+  TN_TRY(contract_launch(w.L1, mk({n}, {1}), mk({n}, {n}), w.B0, mk({n}, {1}), mk({n}, {ld}), R,
+                         mk({n}, {1}), mk({n}, {n}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, w.part,
+                         w.part_bytes, st));
+  // orthogonality of the result: max |Q^T Q - I|, Gram formed in B1
+  TN_TRY(contract_launch(Q, mk({n}, {1}), mk({m}, {n}), Q, mk({m}, {n}), mk({n}, {1}), w.B1,
+                         mk({n}, {1}), mk({n}, {ld}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, w.part,
+                         w.part_bytes, st));
+  hipLaunchKernelGGL(cq_orth, dim3((unsigned)n), dim3(256), 0, st, w.B1, ld, n, w.orth);
+  TN_CHECK_LAUNCH();
+  int fail = 0;
+  unsigned long long ob = 0;
+  TN_HIP(hipMemcpyAsync(&fail, w.fail, sizeof(int), hipMemcpyDeviceToHost, st));
+  TN_HIP(hipMemcpyAsync(&ob, w.orth, sizeof(ob), hipMemcpyDeviceToHost, st));
+  TN_HIP(hipStreamSynchronize(st));
+  double orth;
+  std::memcpy(&orth, &ob, sizeof(orth));
+  if (orth_out) *orth_out = orth;
+  if (fail || !(orth <= orth_tol)) {
+    set_error(TN_ERR_NOCONV,
+              "tn_cholqr: %lld x %lld matrix too ill-conditioned for Cholesky QR "
+              "(pivot failure %d, ||Q^T Q - I||_max = %.3g)",
+              (long long)m, (long long)n, fail, orth);
+    return TN_ERR_NOCONV;
+  }
+  return TN_OK;
+}

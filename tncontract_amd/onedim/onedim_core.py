@@ -604,7 +604,7 @@ def _variational_compress(self, chi, max_iter, initial_guess, tolerance, normali
                 upd = tsr.contract(upd, renv, mps2.right_label, re + "2")
                 upd.replace_label(re + "1", mps1.right_label)
             upd.replace_label([le + "1", mps2.phys_label], [mps1.left_label, mps1.phys_label])
-            Lt, Q = tsr.tensor_lq(upd, mps1.left_label, lq_label=lq)
+            Lt, Q = tsr._lq_rowspace(upd, mps1.left_label, lq_label=lq)
             Q.replace_label(lq + "out", mps1.left_label)
             Lt.replace_label(lq + "in", mps1.right_label)
             mps1[i] = Q

@@ -711,6 +711,47 @@ def tensor_lq(tensor, row_labels, lq_label="lq_"):
     return Lm, Q
 
 
+def _cholqr_device(d, nrow):
+    """Two-pass Cholesky QR (tn_cholqr) of the (rows | cols) view of the FP64
+    tensor d with m >= n: (q m x n, r n x n) on the device, or None when the
+    view is wide, not FP64 or too ill-conditioned for Cholesky QR."""
+    rows, cols, m, n = _matrix_modes(d, nrow)
+    if d.dtype != torch.float64 or m < n or n == 0:
+        return None
+    q = torch.empty((m, n), dtype=d.dtype, device=d.device)
+    r = torch.empty((n, n), dtype=d.dtype, device=d.device)
+    lib = L.lib()
+    ws = L.workspace(lib.tn_cholqr_workspace(m, n))
+    orth = ctypes.c_double(0.0)
+    rc = lib.tn_cholqr(L.ptr(d), ctypes.byref(rows), ctypes.byref(cols), m, n, L.ptr(q), L.ptr(r),
+                       0.0, ctypes.byref(orth), L.ptr(ws), ws.numel(), L.stream_ptr())
+    if rc == L.TN_ERR_NOCONV:
+        return None
+    L.check(rc)
+    return q, r
+
+
+def _lq_rowspace(tensor, row_labels, lq_label="lq_"):
+    """tensor_lq's (L, Q) for a caller that uses only the row space of Q and
+    L Q = tensor: variational_compress (onedim_core.py:560-631), where Q
+    becomes the site and L is absorbed into the neighbour.  Factors by
+    two-pass Cholesky QR on the GPU, so each row of Q (column of L) may differ
+    in sign from Householder's; blocks too ill-conditioned for it, wide ones
+    and non-FP64 ones go through tensor_lq.  Labels and shapes are
+    tensor_lq's (tensor.py:1058-1109)."""
+    col_labels = [x for x in tensor.labels if x not in row_labels]
+    t, nrow = _row_view(tensor, col_labels, move_all=True)
+    res = _cholqr_device(t.data, nrow)
+    if res is None:
+        return tensor_lq(tensor, row_labels, lq_label)
+    q, r = res
+    k = q.shape[1]
+    shape = t.shape
+    Q = Tensor._wrap(q.view(tuple(shape[:nrow]) + (k,)), list(t.labels[:nrow]) + [lq_label + "out"])
+    Lm = Tensor._wrap(r.view((k,) + tuple(shape[nrow:])), [lq_label + "in"] + list(t.labels[nrow:]))
+    return Lm, Q
+
+
 def conjugate(tensor):
     t = tensor.copy()
     t.conjugate()
