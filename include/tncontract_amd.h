@@ -135,19 +135,19 @@ size_t tn_qr_workspace(int64_t m, int64_t n, int dtype);
 int tn_qr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m, int64_t n,
           void* Q, void* R, int dtype, void* ws, size_t ws_bytes, void* stream);
 
-/* ---- Two-pass Cholesky QR (row space of a variational site update) -------
+/* ---- Cholesky QR (row space of a variational site update) ---------------
  * A = Q R for the m x n FP64 matrix A, m >= n (decoded as in tn_qr): Q m x n
- * row-major with orthonormal columns, R n x n row-major upper triangular
- * with a positive diagonal.  Replaces tensor_lq (tensor.py:1058-1109) where
- * variational_compress calls it (onedim_core.py:560-631): there only the row
- * space matters (Q becomes the site, L = R^T is absorbed into the
- * neighbour), so the factors may differ from Householder's by the sign of
- * each row.  Method: G = A^T A = L L^T (blocked Cholesky), Q1 = A L^{-T},
- * then the same on Q1; R = (L1 L2)^T.  *orth_out (host, may be NULL)
- * receives max |Q^T Q - I|.  Returns TN_ERR_NOCONV (outputs unusable) when a
- * first-pass pivot falls below 1e-14 of the largest diagonal of A^T A or the
- * orthogonality exceeds orth_tol (<= 0 -> 1e-12): the caller then uses
- * tn_qr. */
+ * row-major with orthonormal columns, R = Q^T A n x n row-major.  Replaces
+ * tensor_lq (tensor.py:1058-1109) where variational_compress calls it
+ * (onedim_core.py:560-631): there only the row space matters (Q becomes the
+ * site, L = R^T is absorbed into the neighbour), so Q may differ from
+ * Householder's by an orthogonal n x n factor, and R is upper triangular
+ * only up to O(cond(A)^2 eps).  Method: G = A^T A = L L^T (blocked
+ * Cholesky), Q1 = A L^{-T}, one Newton-Schulz step Q = Q1 (3I - Q1^T Q1)/2,
+ * R = Q^T A.  *orth_out (host, may be NULL) receives max |Q^T Q - I|.
+ * Returns TN_ERR_NOCONV (outputs unusable) when a Cholesky pivot falls below
+ * 1e-14 of the largest diagonal of A^T A or the orthogonality exceeds
+ * orth_tol (<= 0 -> 1e-12): the caller then uses tn_qr. */
 size_t tn_cholqr_workspace(int64_t m, int64_t n);
 int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m, int64_t n,
               double* Q, double* R, double orth_tol, double* orth_out, void* ws, size_t ws_bytes,

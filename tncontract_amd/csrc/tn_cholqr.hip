@@ -1,18 +1,19 @@
-// Two-pass Cholesky QR of a tall FP64 matrix: A = Q R with orthonormal Q.
+// Cholesky QR with a Newton-Schulz correction for a tall FP64 matrix:
+// A = Q R with orthonormal Q.
 //
 // The variational sweep (onedim_core.py:560-631) re-orthogonalises one site
 // per update through tensor_lq (tensor.py:1058-1109), whose only use there
 // is the row space: Q becomes the site, L is absorbed into the neighbour, so
 // any A = L Q with orthonormal Q gives the same state.  Householder QR of
-// these 1024 x 512 blocks is a chain of latency-bound panel kernels;
-// Cholesky QR is three GEMM-shaped products per pass plus one small
-// triangular factorisation per 64 columns:
-//   G = A^T A = L L^T,  Q1 = A L^{-T};  the same on Q1 (Q1 = Q2 L2^T);
-//   A = Q2 (L1 L2)^T.
-// The second pass restores orthogonality to rounding when cond(A) is well
-// below eps^{-1/2}; the driver checks ||Q^T Q - I||_max at the end and
-// reports ill-conditioned inputs (TN_ERR_NOCONV) so the caller can fall back
-// to Householder.
+// these 1024 x 512 blocks is a chain of latency-bound panel kernels; here:
+//   G = A^T A = L L^T (blocked Cholesky),  Q1 = A L^{-T}     (Cholesky QR)
+//   Q = Q1 (3 I - Q1^T Q1) / 2                              (Newton-Schulz)
+//   R = Q^T A.
+// Q1^T Q1 = I + E with |E| ~ cond(A)^2 eps, and the correction leaves
+// O(|E|^2): orthogonal to rounding for cond(A) up to ~1e4-1e5.  R is upper
+// triangular up to O(|E|) (R = (I + E/2) L^T + O(|E|^2)).  The driver
+// checks ||Q^T Q - I||_max at the end and reports inputs outside that range
+// (TN_ERR_NOCONV) so the caller can fall back to Householder.
 #include "tn_internal.h"
 #include "tn_prof.h"
 
@@ -26,6 +27,15 @@ int splitk_factor(int64_t M, int64_t N, int64_t K);
 namespace {
 
 constexpr int QB = 64;  // Cholesky block
+
+// 1/sqrt(x) for FP64: the hardware estimate (v_rsq_f64) refined by two
+// Newton steps, y <- y + y (1/2 - x y^2 / 2), each in fused form.
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = fma(y, fma(-hx * y, y, 0.5), y);
+  return fma(y, fma(-hx * y, y, 0.5), y);
+}
 
 // S[n + i, j] = A(i, j): the m x n matrix under the n x n Gram block of the
 // stacked (n + m) x n column-major buffer S (ld = n + m).
@@ -68,10 +78,12 @@ __global__ void __launch_bounds__(256) cq_diag(const double* __restrict__ S, int
   __shared__ double col[QB];
   __shared__ double xrow[QB];
   __shared__ double piv;
-  const int t = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int t = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
   const double pm = *pivmin;
   bool bad = false;
   double r[16], x[16];
+  if (threadIdx.x < QB) col[threadIdx.x] = 0.0;  // read (times zero) before first written
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int c = 16 * q + k;
@@ -92,14 +104,11 @@ __global__ void __launch_bounds__(256) cq_diag(const double* __restrict__ S, int
     } else {
       d = 1.0;
     }
-    const double sq = sqrt(d);
-    const double inv = 1.0 / sq;
+    const double inv = rsq_nr(d), sq = d * inv;
     if (q == qc) {
-      if (t == c) r[kc] = sq;
-      if (t > c) {
-        r[kc] *= inv;
-        col[t] = r[kc];
-      }
+      const double v = t == c ? sq : r[kc] * inv;
+      if (t >= c) r[kc] = v;
+      if (t > c) col[t] = v;
     }
     if (t == c) {
 #pragma unroll
@@ -109,13 +118,12 @@ __global__ void __launch_bounds__(256) cq_diag(const double* __restrict__ S, int
       }
     }
     __syncthreads();
-    if (t > c) {
-      const double l = col[t];
+    const double l = t > c ? col[t] : 0.0;
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (16 * q + k > c) r[k] -= l * col[16 * q + k];
-        x[k] -= l * xrow[16 * q + k];
-      }
+    for (int k = 0; k < 16; ++k) {
+      const double lk = 16 * q + k > c ? l : 0.0;
+      r[k] -= lk * col[16 * q + k];
+      x[k] -= l * xrow[16 * q + k];
     }
   }
 #pragma unroll
@@ -146,34 +154,28 @@ __global__ void __launch_bounds__(256) cq_orth(const double* __restrict__ G, int
   }
 }
 
-// Lo (n x n column-major) = lower triangle of the leading block of F (ld).
-__global__ void cq_lower(const double* __restrict__ F, int64_t ld, int64_t n,
-                         double* __restrict__ Lo) {
+// M = (3 I - G) / 2 in place (n x n, column-major).
+__global__ void cq_ns(double* __restrict__ G, int64_t n) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n * n) return;
   const int64_t c = idx / n, r = idx - c * n;
-  Lo[idx] = r >= c ? F[c * ld + r] : 0.0;
+  G[idx] = (r == c ? 1.5 : 0.0) - 0.5 * G[idx];
 }
 
-// Q (m x n row-major) = F[n:, :] (column-major, ld), through an LDS tile.
-__global__ void __launch_bounds__(256) cq_qout(const double* __restrict__ F, int64_t ld,
-                                               int64_t m, int64_t n, double* __restrict__ Q) {
-  __shared__ double tile[32][33];
-  const int64_t i0 = (int64_t)blockIdx.x * 32, j0 = (int64_t)blockIdx.y * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty 0..7
-  for (int jj = ty; jj < 32; jj += 8) {
-    const int64_t i = i0 + tx, j = j0 + jj;
-    tile[jj][tx] = (i < m && j < n) ? F[j * ld + n + i] : 0.0;
+// The dims of `shape` with the strides of a row-major layout whose last
+// dimension has stride `unit`.
+Modes row_major_like(const Modes& shape, int64_t unit) {
+  Modes r = shape;
+  int64_t st = unit;
+  for (int i = shape.n - 1; i >= 0; --i) {
+    r.strides[i] = st;
+    st *= shape.dims[i];
   }
-  __syncthreads();
-  for (int ii = ty; ii < 32; ii += 8) {
-    const int64_t i = i0 + ii, j = j0 + tx;
-    if (i < m && j < n) Q[i * n + j] = tile[tx][ii];
-  }
+  return r;
 }
 
 struct CqWs {
-  double *B0, *B1, *L1, *X, *pivmin, *part;
+  double *B0, *B1, *M, *X, *pivmin, *part;
   unsigned long long* orth;
   int* fail;
   size_t part_bytes;
@@ -184,14 +186,16 @@ size_t cq_layout(int64_t m, int64_t n, char* base, size_t cap, CqWs* ws) {
   CqWs w{};
   w.B0 = a.take<double>((size_t)((n + m) * n));
   w.B1 = a.take<double>((size_t)((n + m) * n));
-  w.L1 = a.take<double>((size_t)(n * n));
+  w.M = a.take<double>((size_t)(n * n));
   w.X = a.take<double>((size_t)QB * QB);
   w.pivmin = a.take<double>(2);
   w.orth = a.take<unsigned long long>(1);
   w.fail = a.take<int>(1);
-  const int S = std::max(splitk_factor(n, n, m), splitk_factor(n, n, n));
-  w.part_bytes = S > 1 ? (size_t)S * n * n * sizeof(double) : 0;
-  w.part = S > 1 ? a.take<double>((size_t)S * n * n) : nullptr;
+  // split-K partials for the n x n Grams (K = m) and the m x n correction (K = n)
+  const size_t pe = std::max((size_t)splitk_factor(n, n, m) * n * n,
+                             (size_t)splitk_factor(m, n, n) * m * n);
+  w.part_bytes = pe > (size_t)(n * n) ? pe * sizeof(double) : 0;
+  w.part = w.part_bytes ? a.take<double>(pe) : nullptr;
   if (ws) *ws = w;
   return a.used;
 }
@@ -262,25 +266,30 @@ extern "C" int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* 
   hipLaunchKernelGGL(cq_gather, dim3((unsigned)ceil_div(m * n, 256)), dim3(256), 0, st,
                      (const double*)A, rows, cols, m, n, w.B0);
   TN_CHECK_LAUNCH();
-  // pass 1 (B0 -> B1): pivots must stay above 1e-14 of the largest diagonal
-  // of A^T A (cond(A) up to ~1e7)
+  // Cholesky QR (B0 -> B1): pivots must stay above 1e-14 of the largest
+  // diagonal of A^T A; Q1 = B1[n:, :]
   TN_TRY(cq_pass(w.B0, w.B1, m, n, 1e-14, w, st));
-  const unsigned nb = (unsigned)ceil_div(n * n, 256);
-  hipLaunchKernelGGL(cq_lower, dim3(nb), dim3(256), 0, st, w.B1, ld, n, w.L1);
-  // pass 2 (B1 -> B0) on Q1, nearly orthonormal: its Gram's pivots stay near 1
-  TN_TRY(cq_pass(w.B1, w.B0, m, n, 1e-2, w, st));
-  hipLaunchKernelGGL(cq_qout, dim3((unsigned)ceil_div(m, 32), (unsigned)ceil_div(n, 32)),
-                     dim3(256), 0, st, w.B0, ld, m, n, Q);
+  // M = (3 I - Q1^T Q1) / 2
+  TN_TRY(contract_launch(w.B1 + n, mk({n}, {ld}), mk({m}, {1}), w.B1 + n, mk({m}, {1}),
+                         mk({n}, {ld}), w.M, mk({n}, {1}), mk({n}, {n}), TN_F64, TN_PREC_NATIVE,
+                         1.0, 0.0, w.part, w.part_bytes, st));
+  hipLaunchKernelGGL(cq_ns, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, st, w.M, n);
   TN_CHECK_LAUNCH();
-  // R = (L1 L2)^T: the column-major product L1 L2 read row-major
-  TN_TRY(contract_launch(w.L1, mk({n}, {1}), mk({n}, {n}), w.B0, mk({n}, {1}), mk({n}, {ld}), R,
-                         mk({n}, {1}), mk({n}, {n}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, w.part,
+  // Q = Q1 M  (row-major m x n)
+  TN_TRY(contract_launch(w.B1 + n, mk({m}, {1}), mk({n}, {ld}), w.M, mk({n}, {1}), mk({n}, {n}),
+                         Q, mk({m}, {n}), mk({n}, {1}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, w.part,
                          w.part_bytes, st));
-  // orthogonality of the result: max |Q^T Q - I|, Gram formed in B1
-  TN_TRY(contract_launch(Q, mk({n}, {1}), mk({m}, {n}), Q, mk({m}, {n}), mk({n}, {1}), w.B1,
+  // R = Q^T A  (row-major n x n); the contraction launcher pairs the K modes
+  // of both operands and the N modes of B and C mode by mode, so Q's row
+  // index and R's column index are split like A's row and column modes
+  TN_TRY(contract_launch(Q, mk({n}, {1}), row_major_like(rows, n), A, rows, cols, R,
+                         mk({n}, {n}), row_major_like(cols, 1), TN_F64, TN_PREC_NATIVE, 1.0, 0.0,
+                         w.part, w.part_bytes, st));
+  // orthogonality of the result: max |Q^T Q - I|, Gram formed in B0
+  TN_TRY(contract_launch(Q, mk({n}, {1}), mk({m}, {n}), Q, mk({m}, {n}), mk({n}, {1}), w.B0,
                          mk({n}, {1}), mk({n}, {ld}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, w.part,
                          w.part_bytes, st));
-  hipLaunchKernelGGL(cq_orth, dim3((unsigned)n), dim3(256), 0, st, w.B1, ld, n, w.orth);
+  hipLaunchKernelGGL(cq_orth, dim3((unsigned)n), dim3(256), 0, st, w.B0, ld, n, w.orth);
   TN_CHECK_LAUNCH();
   int fail = 0;
   unsigned long long ob = 0;
