@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) cq_pivmin(const double* __restrict__ S, i
 // block, columns 16q..16q+15, in registers, for both the Cholesky
 // elimination and X, which is accumulated by the same column sweep
 // (X <- I; at step c: X[c,:] /= L[c,c], X[t,:] -= L[t,c] X[c,:] for t > c).
-// Two barriers per column.  A pivot not above *pivmin sets *fail and is
+// One barrier per column.  A pivot not above *pivmin sets *fail and is
 // replaced by *pivmin (or 1) so that everything stays finite; the caller
 // discards the result.  Padding rows/columns (>= w) are zero with unit
 // pivots.
@@ -75,27 +75,35 @@ __global__ void __launch_bounds__(256) cq_diag(const double* __restrict__ S, int
                                                const double* __restrict__ pivmin,
                                                double* __restrict__ F, double* __restrict__ X,
                                                int* __restrict__ fail) {
-  __shared__ double col[QB];
-  __shared__ double xrow[QB];
-  __shared__ double piv;
+  // column c of the Schur complement and row c of X, unscaled, double
+  // buffered by the parity of c: one barrier per column
+  __shared__ double colb[2][QB];
+  __shared__ double xb[2][QB];
   const int t = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
   const double pm = *pivmin;
   bool bad = false;
   double r[16], x[16];
-  if (threadIdx.x < QB) col[threadIdx.x] = 0.0;  // read (times zero) before first written
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int c = 16 * q + k;
     r[k] = (t < w && c < w) ? S[(j0 + c) * ld + j0 + t] : 0.0;
     x[k] = c == t ? 1.0 : 0.0;
   }
+  // the column loop is unrolled by 16 only (register indices static), so
+  // the loop body stays resident in the instruction cache
+#pragma unroll 1
+  for (int qc = 0; qc < 4; ++qc)
 #pragma unroll
-  for (int c = 0; c < QB; ++c) {
-    const int qc = c >> 4, kc = c & 15;
-    if (q == qc && t == c) piv = r[kc];
+  for (int kc = 0; kc < 16; ++kc) {
+    const int c = 16 * qc + kc, p = kc & 1;
+    if (q == qc) colb[p][t] = r[kc];
+    if (t == c) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) xb[p][16 * q + k] = x[k];
+    }
     __syncthreads();
-    double d = piv;
+    double d = colb[p][c];
     if (c < w) {
       if (!(d > pm)) {
         bad = true;
@@ -104,26 +112,22 @@ __global__ void __launch_bounds__(256) cq_diag(const double* __restrict__ S, int
     } else {
       d = 1.0;
     }
-    const double inv = rsq_nr(d), sq = d * inv;
+    const double inv = rsq_nr(d);
+    const double l = t > c ? colb[p][t] * inv : 0.0;
     if (q == qc) {
-      const double v = t == c ? sq : r[kc] * inv;
-      if (t >= c) r[kc] = v;
-      if (t > c) col[t] = v;
+      if (t == c) r[kc] = d * inv;
+      if (t > c) r[kc] = l;
     }
     if (t == c) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        x[k] *= inv;
-        xrow[16 * q + k] = x[k];
-      }
+      for (int k = 0; k < 16; ++k) x[k] *= inv;
     }
-    __syncthreads();
-    const double l = t > c ? col[t] : 0.0;
+    const double li = l * inv;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const double lk = 16 * q + k > c ? l : 0.0;
-      r[k] -= lk * col[16 * q + k];
-      x[k] -= l * xrow[16 * q + k];
+      const double lk = 16 * q + k > c ? colb[p][16 * q + k] : 0.0;
+      r[k] -= li * lk;
+      x[k] -= li * xb[p][16 * q + k];
     }
   }
 #pragma unroll
