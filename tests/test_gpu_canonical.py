@@ -208,3 +208,24 @@ def test_apply_gate_layer_matches_sequential_and_oracle(tn, chi):
     same_state(g, r)
     with pytest.raises(ValueError):
         g.apply_gate_layer(tn.Tensor(one, ["a", "b", "c", "d"]), [2, 3])
+
+
+def test_batched_svd_many_single_chunk_jobs(tn):
+    """More (job, pair) items than compute units on the one-launch Jacobi
+    step (every job within one 256-row chunk): 48 graded 256 x 256 matrices
+    and 40 tall 250 x 96 ones, against LAPACK."""
+    from tncontract_amd.batched import batched_svd
+    rng = np.random.default_rng(21)
+    mats = []
+    for i in range(48):
+        u = np.linalg.qr(rng.standard_normal((256, 256)))[0]
+        v = np.linalg.qr(rng.standard_normal((256, 256)))[0]
+        mats.append((u * np.logspace(0, -4 - (i % 5), 256)) @ v.T)
+    mats += [rng.standard_normal((250, 96)) for _ in range(40)]
+    U, S, Vt, sweeps = batched_svd([torch.from_numpy(a).cuda() for a in mats])
+    for a, u, s, vt in zip(mats, U, S, Vt):
+        u, s, vt = u.cpu().numpy(), s.cpu().numpy(), vt.cpu().numpy()
+        ref = np.linalg.svd(a, compute_uv=False)
+        np.testing.assert_allclose(s, ref, rtol=1e-11, atol=1e-12 * ref[0])
+        np.testing.assert_allclose((u * s) @ vt, a, atol=1e-11 * ref[0])
+        np.testing.assert_allclose(vt @ vt.T, np.eye(len(s)), atol=1e-11)

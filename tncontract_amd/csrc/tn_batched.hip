@@ -241,6 +241,72 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
   }
 }
 
+// One Jacobi step for batches whose jobs fit one row chunk (L <= BCH and
+// k <= BCH): the block of each (job, pair) eig item forms the pair's
+// 32 x 32 Gram over all L rows of W in LDS, runs the pair eigensolve and
+// rotates the pair's rows of W and then of Z.  One block owns the pair's
+// vectors for the whole step (a second block rotating them while this one
+// still reads them for its Gram would race).  The same arithmetic as
+// bjac_gram + bjac_eig + bjac_apply, in one launch and without the partial
+// Grams and rotations passing through global memory.
+__global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, double* __restrict__ Zb,
+                                                  const BJob* __restrict__ jobs,
+                                                  const int* __restrict__ e_job,
+                                                  const int* __restrict__ e_pair,
+                                                  const int* __restrict__ done, int g,
+                                                  unsigned long long* __restrict__ offmax,
+                                                  unsigned long long* __restrict__ work) {
+  __shared__ double Ps[64][BJ2 + 1];
+  __shared__ EigLds sm;
+  __shared__ ApplyTile Tw;
+  const int item = blockIdx.x, tid = threadIdx.x;
+  const int j = e_job[item];
+  if (done[j]) return;
+  const BJob J = jobs[j];
+  const int s = g % (J.nblk - 1);
+  const int pi = e_pair[item];
+  const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
+  const int wave = tid >> 6, lane = tid & 63;
+  const int ti = wave >> 1, tj = wave & 1;
+  double* W = Wv + J.wv;
+  double* Z = Zb + J.z;
+  double x[BJ2], z[BJ2];
+  apply_tile_fetch(W, J.L, J.k, 0, J.L, I, K2, x);  // in flight during the Gram and rounds
+  apply_tile_fetch(Z, J.k, J.k, 0, J.k, I, K2, z);
+  f64x4b acc = {0, 0, 0, 0};
+  for (int64_t rb = 0; rb < J.L; rb += 64) {
+    for (int e = tid; e < 64 * BJ2; e += 256) {
+      int row = e & 63, col = e >> 6;
+      int64_t v = col < BJB ? (int64_t)I * BJB + col : (int64_t)K2 * BJB + (col - BJB);
+      int64_t r = rb + row;
+      Ps[row][col] = (v < J.k && r < J.L) ? W[v * J.L + r] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k0 = 0; k0 < 64; k0 += 4) {
+      double a = Ps[k0 + (lane >> 4)][16 * ti + (lane & 15)];
+      double b = Ps[k0 + (lane >> 4)][16 * tj + (lane & 15)];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) sm.G[0][16 * ti + (lane >> 4) + 4 * e][16 * tj + (lane & 15)] = acc[e];
+  for (int e = tid; e < BJ2 * BJ2; e += 256) sm.Q[0][e / BJ2][e % BJ2] = (e / BJ2 == e % BJ2) ? 1.0 : 0.0;
+  __syncthreads();
+  const double off = eig_off(sm);
+  if (tid == 0) atomicMax(offmax + j, dbits(off));
+  if (off <= J.tol) return;
+  if (work && tid == 0) {
+    atomicAdd(work, (unsigned long long)J.L);
+    atomicAdd(work + 1, 1ull);
+    atomicAdd(work + 2, (unsigned long long)(J.L + J.k));
+  }
+  const int cur = eig_rotate(sm, J.tol);
+  apply_tile_finish(W, J.L, J.k, 0, J.L, I, K2, x, sm.Q[cur], Tw);
+  apply_tile_finish(Z, J.k, J.k, 0, J.k, I, K2, z, sm.Q[cur], Tw);
+}
+
 // After global step g: retire jobs whose sweep just ended with off <= tol.
 __global__ void bjac_sweep_end(const BJob* __restrict__ jobs, int njobs, int g, int max_sweeps,
                                int* __restrict__ done, int* __restrict__ sweeps,
@@ -665,10 +731,18 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   hipLaunchKernelGGL(binit_identity, dim3((unsigned)ceil_div((int64_t)maxk * maxk, 256), njobs),
                      dim3(256), 0, st, dZ, dbj);
   TN_CHECK_LAUNCH();
+  // every job within one row chunk: the one-launch step
+  bool fused = true;
+  for (int j = 0; j < njobs; ++j) fused = fused && bj[j].L <= BCH && bj[j].k <= BCH;
   const int poll = std::max(8, maxsteps);
   int g = 0;
   for (; g < maxsteps * max_sweeps; ++g) {
     // work is counted on the device (cnt) and added once at the end
+    if (fused) {
+      ProfScope p(PROF_JAC_EIG, st, 0.0, 0.0);
+      hipLaunchKernelGGL(bjac_fused, dim3(nei), dim3(256), 0, st, dWv, dZ, dbj, det, det + nei,
+                         ddone, g, doff, cnt);
+    } else {
     {
       ProfScope p(PROF_JAC_GRAM, st, 0.0, 0.0);
       hipLaunchKernelGGL(bjac_gram, dim3(ngi), dim3(256), 0, st, dWv, dbj, dgt, dgt + ngi,
@@ -683,6 +757,7 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
       ProfScope p(PROF_JAC_APPLY, st, 0.0, 0.0);
       hipLaunchKernelGGL(bjac_apply, dim3(nai), dim3(256), 0, st, dWv, dZ, dbj, dat, dat + nai,
                          dat + 2 * nai, ddone, g, dQ, dskip, cnt);
+    }
     }
     hipLaunchKernelGGL(bjac_sweep_end, dim3((unsigned)ceil_div(njobs, 256)), dim3(256), 0, st, dbj,
                        njobs, g, max_sweeps, ddone, dsweeps, doff, dactive);
