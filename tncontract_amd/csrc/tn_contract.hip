@@ -62,12 +62,14 @@ template <> struct PrecTraits<P_BF16X3> { using T = float; static constexpr int 
 // k-tile needs one barrier.  SIMPLE: every mode group has at most one mode
 // (the common case after merging), so operand offsets are index * stride
 // computed in registers instead of read from the per-tile row/column tables.
-template <int P, bool SIMPLE, int AKF, int BNF>
+template <int P, bool SIMPLE, int AKF, int BNF, int TM>
 __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
+  constexpr int BM = TM, BN = TM;   // tile edge (128, or 64 for few-tile shapes)
+  constexpr int FRG = TM / 32;      // 16 x 16 fragments per wave and dimension
   using T = typename PrecTraits<P>::T;
   constexpr int BK = PrecTraits<P>::BK;
   constexpr int LDA = BM + 2, LDB = BN + 2;
-  constexpr int EPT_A = BM * BK / NT, EPT_B = BN * BK / NT;  // 8 each
+  constexpr int EPT_A = BM * BK / NT, EPT_B = BN * BK / NT;  // 8 each (4 at TM = 64)
   __shared__ T As[2][BK * LDA];
   __shared__ T Bs[2][BK * LDB];
   __shared__ int64_t rowA[SIMPLE ? 1 : BM], colB[SIMPLE ? 1 : BN], rowC[SIMPLE ? 1 : BM],
@@ -142,15 +144,15 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
   };
 
   const int wave = tid >> 6, lane = tid & 63;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int wr = (wave >> 1) * (TM / 2), wc = (wave & 1) * (TM / 2);
   const int lr = lane & 15, lk = lane >> 4;
 
   using Acc = typename std::conditional<P == P_F64, f64x4, f32x4>::type;
-  Acc acc[4][4];
+  Acc acc[FRG][FRG];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FRG; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+    for (int j = 0; j < FRG; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
   const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   if (nk > 0) {
@@ -167,15 +169,15 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
     // four and pushes the FP64 kernel past the two-wave register budget)
 #pragma unroll 2
     for (int ks = 0; ks < BK; ks += 4) {
-      T af[4], bf[4];
+      T af[FRG], bf[FRG];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = as[(ks + lk) * LDA + wr + 16 * i + lr];
+      for (int i = 0; i < FRG; ++i) af[i] = as[(ks + lk) * LDA + wr + 16 * i + lr];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = bs[(ks + lk) * LDB + wc + 16 * j + lr];
+      for (int j = 0; j < FRG; ++j) bf[j] = bs[(ks + lk) * LDB + wc + 16 * j + lr];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < FRG; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < FRG; ++j) {
           if constexpr (P == P_F64)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
           else
@@ -192,9 +194,9 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
   // FP32 at row 4*(l/16)+e; column l%16 in both.
   const T alpha = (T)g.alpha, beta = (T)g.beta;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FRG; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < FRG; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         int r = wr + 16 * i + (P == P_F64 ? (lk + 4 * e) : (4 * lk + e));
@@ -578,7 +580,11 @@ int contract_launch(const void* A, const Modes& a_m, const Modes& a_k, const voi
   g.kchunk = S > 1 ? ceil_div(ceil_div(K, S), 32) * 32 : K;
   if (S > 1) S = (int)ceil_div(K, g.kchunk);
   g.part = S > 1 ? splitk_ws : nullptr;
-  dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)S);
+  // FP64 products with few 128 x 128 tiles and no K split (short K): 64 x 64
+  // tiles, four times the workgroups
+  const bool t64 = prec == P_F64 && S == 1 && tiles < 128;
+  const int tm = t64 ? 64 : BM;
+  dim3 grid((unsigned)ceil_div(N, tm), (unsigned)ceil_div(M, tm), (unsigned)S);
   const size_t esz_in = dtype == TN_F64 ? 8 : 4;
   ProfScope prof(prec == P_F64 ? PROF_GEMM_F64 : (prec == P_F32 ? PROF_GEMM_F32 : PROF_GEMM_BF16X3),
                  st, 2.0 * (double)M * (double)N * (double)K,
@@ -594,21 +600,32 @@ int contract_launch(const void* A, const Modes& a_m, const Modes& a_k, const voi
       (double)(M + BM) * st1(g.cm) + (double)(N + BN) * st1(g.cn) < lim;
   // single-mode FP64 operands (the hot case): the tile-load mapping of each
   // operand is a template flag, so only one index mapping is live
-  if (prec == P_F64 && simple) {
+  if (prec == P_F64 && simple && t64) {
     if (g.a_kfast && g.b_nfast)
-      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 1, 1>), grid, dim3(NT), 0, st, g);
+      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 1, 1, 64>), grid, dim3(NT), 0, st, g);
     else if (g.a_kfast)
-      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 1, 0>), grid, dim3(NT), 0, st, g);
+      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 1, 0, 64>), grid, dim3(NT), 0, st, g);
     else if (g.b_nfast)
-      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 0, 1>), grid, dim3(NT), 0, st, g);
+      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 0, 1, 64>), grid, dim3(NT), 0, st, g);
     else
-      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 0, 0>), grid, dim3(NT), 0, st, g);
-  } else if (prec == P_F64)
-    hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, false, -1, -1>), grid, dim3(NT), 0, st, g);
+      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 0, 0, 64>), grid, dim3(NT), 0, st, g);
+  } else if (prec == P_F64 && simple) {
+    if (g.a_kfast && g.b_nfast)
+      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 1, 1, 128>), grid, dim3(NT), 0, st, g);
+    else if (g.a_kfast)
+      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 1, 0, 128>), grid, dim3(NT), 0, st, g);
+    else if (g.b_nfast)
+      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 0, 1, 128>), grid, dim3(NT), 0, st, g);
+    else
+      hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, true, 0, 0, 128>), grid, dim3(NT), 0, st, g);
+  } else if (prec == P_F64 && t64)
+    hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, false, -1, -1, 64>), grid, dim3(NT), 0, st, g);
+  else if (prec == P_F64)
+    hipLaunchKernelGGL((gemm_mfma_kernel<P_F64, false, -1, -1, 128>), grid, dim3(NT), 0, st, g);
   else if (prec == P_F32 && simple)
-    hipLaunchKernelGGL((gemm_mfma_kernel<P_F32, true, -1, -1>), grid, dim3(NT), 0, st, g);
+    hipLaunchKernelGGL((gemm_mfma_kernel<P_F32, true, -1, -1, 128>), grid, dim3(NT), 0, st, g);
   else if (prec == P_F32)
-    hipLaunchKernelGGL((gemm_mfma_kernel<P_F32, false, -1, -1>), grid, dim3(NT), 0, st, g);
+    hipLaunchKernelGGL((gemm_mfma_kernel<P_F32, false, -1, -1, 128>), grid, dim3(NT), 0, st, g);
   else if (prec == P_BF16X3)
     hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(NT), 0, st, g);
   else {
