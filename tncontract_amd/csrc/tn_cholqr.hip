@@ -240,28 +240,15 @@ int cq_pass(double* S, double* F, int64_t m, int64_t n, double thr, const CqWs& 
 }
 
 }  // namespace
-}  // namespace tn
 
-using namespace tn;
-
-extern "C" size_t tn_cholqr_workspace(int64_t m, int64_t n) {
+size_t cholqr_ws_bytes(int64_t m, int64_t n) {
   if (m <= 0 || n <= 0) return 256;
   return cq_layout(m, n, nullptr, 0, nullptr) + 256;
 }
 
-extern "C" int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m,
-                         int64_t n, double* Q, double* R, double orth_tol, double* orth_out,
-                         void* ws, size_t ws_bytes, void* stream) {
-  Modes rows = to_modes(a_rows), cols = to_modes(a_cols);
-  TN_REQUIRE(m >= 0 && n >= 0 && m >= n, "tn_cholqr: needs m >= n >= 0 (got %lld x %lld)",
-             (long long)m, (long long)n);
-  TN_REQUIRE(modes_size(rows) == m && modes_size(cols) == n, "tn_cholqr: modes do not match m x n");
-  if (orth_out) *orth_out = 0.0;
-  if (n == 0) return TN_OK;
-  TN_REQUIRE(A && Q && R, "tn_cholqr: null pointer");
-  TN_REQUIRE(ws && ws_bytes >= tn_cholqr_workspace(m, n), "tn_cholqr: workspace too small");
-  if (orth_tol <= 0.0) orth_tol = 1e-12;
-  hipStream_t st = as_stream(stream);
+int cholqr_run(const void* A, const Modes& rows, const Modes& cols, int64_t m, int64_t n,
+               double* Q, int64_t q_rs, int64_t q_cs, double* R, double orth_tol,
+               double* orth_out, void* ws, size_t ws_bytes, hipStream_t st) {
   CqWs w{};
   cq_layout(m, n, (char*)ws, ws_bytes, &w);
   const int64_t ld = n + m;
@@ -279,20 +266,20 @@ extern "C" int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* 
                          1.0, 0.0, w.part, w.part_bytes, st));
   hipLaunchKernelGGL(cq_ns, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, st, w.M, n);
   TN_CHECK_LAUNCH();
-  // Q = Q1 M  (row-major m x n)
+  // Q = Q1 M  (element (i, j) at i * q_rs + j * q_cs)
   TN_TRY(contract_launch(w.B1 + n, mk({m}, {1}), mk({n}, {ld}), w.M, mk({n}, {1}), mk({n}, {n}),
-                         Q, mk({m}, {n}), mk({n}, {1}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, w.part,
-                         w.part_bytes, st));
+                         Q, mk({m}, {q_rs}), mk({n}, {q_cs}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0,
+                         w.part, w.part_bytes, st));
   // R = Q^T A  (row-major n x n); the contraction launcher pairs the K modes
   // of both operands and the N modes of B and C mode by mode, so Q's row
   // index and R's column index are split like A's row and column modes
-  TN_TRY(contract_launch(Q, mk({n}, {1}), row_major_like(rows, n), A, rows, cols, R,
+  TN_TRY(contract_launch(Q, mk({n}, {q_cs}), row_major_like(rows, q_rs), A, rows, cols, R,
                          mk({n}, {n}), row_major_like(cols, 1), TN_F64, TN_PREC_NATIVE, 1.0, 0.0,
                          w.part, w.part_bytes, st));
   // orthogonality of the result: max |Q^T Q - I|, Gram formed in B0
-  TN_TRY(contract_launch(Q, mk({n}, {1}), mk({m}, {n}), Q, mk({m}, {n}), mk({n}, {1}), w.B0,
-                         mk({n}, {1}), mk({n}, {ld}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0, w.part,
-                         w.part_bytes, st));
+  TN_TRY(contract_launch(Q, mk({n}, {q_cs}), mk({m}, {q_rs}), Q, mk({m}, {q_rs}),
+                         mk({n}, {q_cs}), w.B0, mk({n}, {1}), mk({n}, {ld}), TN_F64,
+                         TN_PREC_NATIVE, 1.0, 0.0, w.part, w.part_bytes, st));
   hipLaunchKernelGGL(cq_orth, dim3((unsigned)n), dim3(256), 0, st, w.B0, ld, n, w.orth);
   TN_CHECK_LAUNCH();
   int fail = 0;
@@ -311,4 +298,26 @@ extern "C" int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* 
     return TN_ERR_NOCONV;
   }
   return TN_OK;
+}
+
+}  // namespace tn
+
+using namespace tn;
+
+extern "C" size_t tn_cholqr_workspace(int64_t m, int64_t n) { return cholqr_ws_bytes(m, n); }
+
+extern "C" int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m,
+                         int64_t n, double* Q, double* R, double orth_tol, double* orth_out,
+                         void* ws, size_t ws_bytes, void* stream) {
+  Modes rows = to_modes(a_rows), cols = to_modes(a_cols);
+  TN_REQUIRE(m >= 0 && n >= 0 && m >= n, "tn_cholqr: needs m >= n >= 0 (got %lld x %lld)",
+             (long long)m, (long long)n);
+  TN_REQUIRE(modes_size(rows) == m && modes_size(cols) == n, "tn_cholqr: modes do not match m x n");
+  if (orth_out) *orth_out = 0.0;
+  if (n == 0) return TN_OK;
+  TN_REQUIRE(A && Q && R, "tn_cholqr: null pointer");
+  TN_REQUIRE(ws && ws_bytes >= tn_cholqr_workspace(m, n), "tn_cholqr: workspace too small");
+  if (orth_tol <= 0.0) orth_tol = 1e-12;
+  return cholqr_run(A, rows, cols, m, n, Q, n, 1, R, orth_tol, orth_out, ws, ws_bytes,
+                    as_stream(stream));
 }

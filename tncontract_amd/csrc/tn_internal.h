@@ -31,4 +31,13 @@ int qr_factor(double* W, int64_t m, int64_t n, const QrWs& ws, hipStream_t st);
 int qr_form_q(const double* W, int64_t m, int64_t n, const QrWs& ws, double* Qw, hipStream_t st,
               const double* init);
 
+// Cholesky QR with a Newton-Schulz correction (tn_cholqr.hip): A (m x n,
+// m >= n, FP64, decoded through rows/cols) = Q R, Q element (i, j) at
+// i * q_rs + j * q_cs, R = Q^T A row-major n x n.  Synchronises the stream;
+// TN_ERR_NOCONV when a pivot fails or max |Q^T Q - I| > orth_tol.
+size_t cholqr_ws_bytes(int64_t m, int64_t n);
+int cholqr_run(const void* A, const Modes& rows, const Modes& cols, int64_t m, int64_t n,
+               double* Q, int64_t q_rs, int64_t q_cs, double* R, double orth_tol,
+               double* orth_out, void* ws, size_t ws_bytes, hipStream_t st);
+
 }  // namespace tn

@@ -254,6 +254,8 @@ struct SvdWs {
   int *skip, *perm, *vp;
   unsigned long long* off;
   QrWs qr;
+  char* cq;  // Cholesky QR workspace
+  size_t cq_bytes;
   int precond;
 };
 
@@ -283,6 +285,8 @@ static size_t svd_layout(int64_t m, int64_t n, char* base, size_t cap, SvdWs* ou
     size_t qbytes = qr_layout(L, k, nullptr, 0, nullptr) + 256;
     char* qbase = a.take<char>(qbytes);
     if (base) qr_layout(L, k, qbase, qbytes, &w.qr);
+    w.cq_bytes = cholqr_ws_bytes(L, k);
+    w.cq = a.take<char>(w.cq_bytes);
   }
   if (out) *out = w;
   return a.used;
@@ -439,19 +443,39 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
     hipLaunchKernelGGL(permute_vectors, dim3((unsigned)ceil_div(L * k, 256)), dim3(256), 0, st,
                        w.Wv, L, k, w.vp, w.QZ);
     TN_CHECK_LAUNCH();
-    TN_TRY(qr_factor(w.QZ, L, k, w.qr, st));
-    hipLaunchKernelGGL(r_rows_kernel, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, w.QZ,
-                       L, k, w.R);
-    TN_CHECK_LAUNCH();
+    // The factorisation only has to give an orthonormal Q with X' = Q R
+    // (any such R has the singular values and right vectors of X'): Cholesky
+    // QR with a Newton-Schulz correction (a few GEMMs; Q into Wv, R row-major,
+    // i.e. as row vectors) when X' is well enough conditioned for it to reach
+    // orthogonality 1e-13, blocked Householder otherwise.
+    const int cq = cholqr_run(w.QZ, mk({L}, {1}), mk({k}, {L}), L, k, w.Wv, 1, L, w.R, 1e-13,
+                              nullptr, w.cq, w.cq_bytes, st);
+    if (cq != TN_OK && cq != TN_ERR_NOCONV) return cq;
+    const bool householder = cq != TN_OK;
+    if (householder) {
+      TN_TRY(qr_factor(w.QZ, L, k, w.qr, st));
+      hipLaunchKernelGGL(r_rows_kernel, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st,
+                         w.QZ, L, k, w.R);
+      TN_CHECK_LAUNCH();
+    }
     if (tol <= 0.0) tol = std::max(1e-15, (double)k * 2.220446049250313e-16);
     TN_TRY(jacobi_run(w.R, k, w.Z, k, w, tol, max_sweeps, sweeps_out, &converged, st));
-    TN_TRY(qr_form_q(w.QZ, L, k, w.qr, w.Wv, st, w.Z));
+    double* QZv;  // Q Z (L x k, vector-contiguous)
+    if (householder) {
+      TN_TRY(qr_form_q(w.QZ, L, k, w.qr, w.Wv, st, w.Z));
+      QZv = w.Wv;
+    } else {
+      TN_TRY(contract_launch(w.Wv, mk({L}, {1}), mk({k}, {L}), w.Z, mk({k}, {1}), mk({k}, {k}),
+                             w.QZ, mk({L}, {1}), mk({k}, {L}), TN_F64, TN_PREC_NATIVE, 1.0, 0.0,
+                             nullptr, 0, st));
+      QZv = w.QZ;
+    }
     orth_src = w.R;
     orth_len = k;
     // left vectors of X: Q Z (orthonormal); right vectors of X: rows of R
     // rotated, whose components are in pivot order
-    if (tall) { Uvec = w.Wv; ldu = L; u_scaled = 0; Vvec = w.R; ldv = k; v_scaled = 1; vpm = w.vp; }
-    else { Uvec = w.R; ldu = k; u_scaled = 1; Vvec = w.Wv; ldv = L; v_scaled = 0; upm = w.vp; }
+    if (tall) { Uvec = QZv; ldu = L; u_scaled = 0; Vvec = w.R; ldv = k; v_scaled = 1; vpm = w.vp; }
+    else { Uvec = w.R; ldu = k; u_scaled = 1; Vvec = QZv; ldv = L; v_scaled = 0; upm = w.vp; }
   } else {
     // LAPACK dgesvj convergence threshold: (vector length) * eps
     if (tol <= 0.0) tol = std::max(1e-15, (double)L * 2.220446049250313e-16);
