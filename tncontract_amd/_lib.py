@@ -310,6 +310,19 @@ def diag_inverse(a):
     return out, info.cpu().numpy()
 
 
+def diag_inverse_async(a):
+    """diag_inverse without the host read: (inverse, device info [nonzero
+    off-diagonal count, zero diagonal count]), both ordered on the current
+    stream."""
+    n = a.shape[0]
+    out = torch.empty((n, n), dtype=a.dtype, device=a.device)
+    info = torch.zeros(2, dtype=torch.float64, device=a.device)
+    ws = workspace(lib().tn_diag_inverse_workspace(n))
+    check(lib().tn_diag_inverse(ptr(a), n, a.stride(0), a.stride(1), ptr(out), dtype_code(a),
+                                ptr(info), ptr(ws), ws.numel(), stream_ptr()))
+    return out, info
+
+
 def diag_embed(s, dtype=torch.float64):
     """k x k diagonal matrix from a device float64 vector s."""
     k = s.numel()

@@ -11,6 +11,7 @@ reference itself reads.
 from __future__ import annotations
 
 import numpy as np
+import torch
 
 from tncontract_amd import _lib as L
 from tncontract_amd import tensor as tsr
@@ -395,6 +396,28 @@ class MatrixProductStateCanonical(OneDimensionalTensorNetwork):
         t.inv()
         return t
 
+    def _layer_inverses(self, positions):
+        """{pos: self._inverse(pos)} with one host synchronisation: the
+        diagonal inverses are enqueued back to back and their checks read
+        together; a Lambda that is not diagonal, or is singular, goes through
+        Tensor.inv (which raises LinAlgError for a singular one)."""
+        pending, out = [], {}
+        for pos in positions:
+            d = self[pos].data
+            if d.dim() == 2 and d.shape[0] == d.shape[1] and d.shape[0] > 0:
+                inv_d, info = L.diag_inverse_async(d)
+                pending.append((pos, inv_d, info))
+            else:
+                out[pos] = self._inverse(pos)
+        if pending:
+            infos = torch.stack([p[2] for p in pending]).cpu().numpy()
+            for (pos, inv_d, _), (offdiag, zeros) in zip(pending, infos):
+                if offdiag == 0 and zeros == 0:
+                    out[pos] = tsr.Tensor._wrap(inv_d, list(self[pos].labels))
+                else:
+                    out[pos] = self._inverse(pos)
+        return out
+
     def compress_bond(self, singular_site, chi=None, threshold=1e-15):
         """onedim_core.py:1067-1095.  As in the reference, the early-exit test
         reads singulardim(singular_site(n)), i.e. position 4n."""
@@ -489,20 +512,29 @@ class MatrixProductStateCanonical(OneDimensionalTensorNetwork):
             shape = tuple(v.shape)
             m, n = int(np.prod(shape[:nrow])), int(np.prod(shape[nrow:]))
             blocks.append((k, go, list(v.labels), shape, nrow, L.materialize(v.data).view(m, n)))
+        # the outer Lambdas' inverses (read-only in the layer, shared by
+        # neighbouring bonds) before anything is written, as apply_gate takes
+        # them before its writes
+        inv = self._layer_inverses(sorted({self.physical_site(k) - 1 + o
+                                           for k in ks for o in (0, 4)}))
         U, S, Vt, _ = batched_svd([b[5] for b in blocks])
-        for (k, go, labels, shape, nrow, _), u, s, vt in zip(blocks, U, S, Vt):
-            sv = s.cpu().numpy()
+        # every bond's singular values in one device-to-host copy, so the
+        # updates below are enqueued without waiting on the device
+        svals = torch.cat(S).cpu().numpy() if S else np.zeros(0)
+        offs = np.cumsum([0] + [s.numel() for s in S])
+        for (k, go, labels, shape, nrow, _), u, s, vt, o in zip(blocks, U, S, Vt, offs):
+            sv = svals[o:o + s.numel()]
             kk = int(np.count_nonzero((sv[:chi] if chi else sv) > threshold))
             Ut = tsr.Tensor._wrap(L.contiguous(u[:, :kk]).view(shape[:nrow] + (kk,)),
                                   labels[:nrow] + ["svd_in"])
             Vm = tsr.Tensor._wrap(vt[:kk].view((kk,) + shape[nrow:]), ["svd_out"] + labels[nrow:])
-            Sm = tsr.Tensor._wrap(L.diag_embed(_upload(sv[:kk]), u.dtype),
+            Sm = tsr.Tensor._wrap(L.diag_embed(s[:kk], u.dtype),
                                   [self.left_label, self.right_label])
             Ut.replace_label(["svd_in", go[0]], [self.right_label, self.phys_label])
             Vm.replace_label(["svd_out", go[1]], [self.left_label, self.phys_label])
             start = self.physical_site(k) - 1
-            S1_inv = self._inverse(start)
-            S2_inv = self._inverse(start + 4)
+            S1_inv = inv[start]
+            S2_inv = inv[start + 4]
             self[start + 1] = S1_inv[self.right_label, ] * Ut[self.left_label, ]
             self[start + 2] = Sm
             self[start + 3] = Vm[self.right_label, ] * S2_inv[self.left_label, ]
