@@ -259,8 +259,13 @@ struct SvdWs {
   int precond;
 };
 
-// Precondition with a QR when the vectors are much longer than their count.
-static bool use_precond(int64_t L, int64_t k) { return k >= 32 && L >= 2 * k; }
+// Precondition with a QR (static norm pivot, then Jacobi on the rows of R)
+// whenever there are at least 32 vectors: besides shortening long vectors, it
+// grades the problem, so square matrices with spread spectra (bond matrices)
+// need far fewer sweeps (22 -> 8 for a 128 x 128 matrix with singular values
+// from 1 to 1e-6, 10 -> 9 for a Gaussian one, in a simulation of this block
+// Jacobi).
+static bool use_precond(int64_t L, int64_t k) { return k >= 32 && L >= k; }
 
 static size_t svd_layout(int64_t m, int64_t n, char* base, size_t cap, SvdWs* out) {
   const int64_t k = std::min(m, n), L = std::max(m, n);
