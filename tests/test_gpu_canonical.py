@@ -229,3 +229,20 @@ def test_batched_svd_many_single_chunk_jobs(tn):
         np.testing.assert_allclose(s, ref, rtol=1e-11, atol=1e-12 * ref[0])
         np.testing.assert_allclose((u * s) @ vt, a, atol=1e-11 * ref[0])
         np.testing.assert_allclose(vt @ vt.T, np.eye(len(s)), atol=1e-11)
+
+
+def test_singular_lambda_raises_like_np_linalg_inv(tn):
+    """A zero singular value next to the gate's bond: the reference's
+    inverse of the outer Lambda (np.linalg.inv, tensor.py:487-488) raises
+    LinAlgError, for one gate and for a layer, before the state changes."""
+    g, _ = both_canonical(tn, 6, 2, 4, seed=13)
+    lam = g[2].numpy().copy()
+    lam[-1, -1] = 0.0
+    g[2] = tn.Tensor(lam, list(g[2].labels))
+    before = [t.numpy().copy() for t in g]
+    G = np.eye(4).reshape(2, 2, 2, 2)
+    with pytest.raises(np.linalg.LinAlgError):
+        g.apply_gate(tn.Tensor(G, ["o0", "o1", "i0", "i1"]), 1)
+    with pytest.raises(np.linalg.LinAlgError):
+        g.apply_gate_layer(tn.Tensor(G, ["o0", "o1", "i0", "i1"]), [1, 3])
+    assert all(np.array_equal(a, t.numpy()) for a, t in zip(before, g))

@@ -427,8 +427,8 @@ class MatrixProductStateCanonical(OneDimensionalTensorNetwork):
         start, end = site - 2, site + 2
         self[end - 1].prime_label(self.phys_label)
         t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
-        S1_inv = self._inverse(start)
-        S2_inv = self._inverse(end)
+        inv = self._layer_inverses([start, end])
+        S1_inv, S2_inv = inv[start], inv[end]
         U, S, V = tsr.truncated_svd(t, [self.phys_label, self.left_label], chi=chi,
                                     threshold=threshold, absorb_singular_values=None)
         U.replace_label("svd_in", self.right_label)
@@ -466,8 +466,8 @@ class MatrixProductStateCanonical(OneDimensionalTensorNetwork):
         end = self.physical_site(firstsite + nsites - 1) + 1
         t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
         t = tsr.contract(t, gate, self.phys_label, gate_inputs)
-        S1_inv = self._inverse(start)
-        S2_inv = self._inverse(end)
+        inv = self._layer_inverses([start, end])
+        S1_inv, S2_inv = inv[start], inv[end]
         if nsites == 1:
             t.replace_label([gate_outputs[0]], [self.phys_label])
             t = S1_inv[self.right_label, ] * t[self.left_label, ]
@@ -547,8 +547,8 @@ class MatrixProductStateCanonical(OneDimensionalTensorNetwork):
         end = self.physical_site(i + 1) + 1
         self[start + 1].prime_label(self.phys_label)
         t = contract_virtual_indices(self, start, end + 1, periodic_boundaries=False)
-        S1_inv = self._inverse(start)
-        S2_inv = self._inverse(end)
+        inv = self._layer_inverses([start, end])
+        S1_inv, S2_inv = inv[start], inv[end]
         U, S, V = tsr.truncated_svd(t, [self.left_label, self.phys_label], chi=chi,
                                     threshold=threshold, absorb_singular_values=None)
         V.unprime_label(self.phys_label)
