@@ -27,6 +27,7 @@ int splitk_factor(int64_t M, int64_t N, int64_t K);
 namespace {
 
 constexpr int QB = 64;  // Cholesky block
+constexpr int QC = 8;   // columns of the block per wave (eight waves)
 
 // 1/sqrt(x) for FP64: the hardware estimate (v_rsq_f64) refined by two
 // Newton steps, y <- y + y (1/2 - x y^2 / 2), each in fused form.
@@ -62,15 +63,15 @@ __global__ void __launch_bounds__(256) cq_pivmin(const double* __restrict__ S, i
 
 // Factor the w x w diagonal block at (j0, j0) of S (column-major, ld):
 // L11 (lower, zeros above) into F at the same position and X = L11^{-1}
-// (QB x QB, column-major).  256 threads: thread (q, t) keeps row t of the
-// block, columns 16q..16q+15, in registers, for both the Cholesky
+// (QB x QB, column-major).  512 threads: thread (q, t) keeps row t of the
+// block, columns QC q .. QC q + QC - 1, in registers, for both the Cholesky
 // elimination and X, which is accumulated by the same column sweep
 // (X <- I; at step c: X[c,:] /= L[c,c], X[t,:] -= L[t,c] X[c,:] for t > c).
 // One barrier per column.  A pivot not above *pivmin sets *fail and is
 // replaced by *pivmin (or 1) so that everything stays finite; the caller
 // discards the result.  Padding rows/columns (>= w) are zero with unit
 // pivots.
-__global__ void __launch_bounds__(256) cq_diag(const double* __restrict__ S, int64_t ld,
+__global__ void __launch_bounds__(512) cq_diag(const double* __restrict__ S, int64_t ld,
                                                int64_t j0, int w,
                                                const double* __restrict__ pivmin,
                                                double* __restrict__ F, double* __restrict__ X,
@@ -83,56 +84,60 @@ __global__ void __launch_bounds__(256) cq_diag(const double* __restrict__ S, int
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
   const double pm = *pivmin;
   bool bad = false;
-  double r[16], x[16];
+  double r[QC], x[QC];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int c = 16 * q + k;
+  for (int k = 0; k < QC; ++k) {
+    const int c = QC * q + k;
     r[k] = (t < w && c < w) ? S[(j0 + c) * ld + j0 + t] : 0.0;
     x[k] = c == t ? 1.0 : 0.0;
   }
-  // the column loop is unrolled by 16 only (register indices static), so
+  // the column loop is unrolled by QC only (register indices static), so
   // the loop body stays resident in the instruction cache
 #pragma unroll 1
-  for (int qc = 0; qc < 4; ++qc)
+  for (int qc = 0; qc < QB / QC; ++qc)
 #pragma unroll
-  for (int kc = 0; kc < 16; ++kc) {
-    const int c = 16 * qc + kc, p = kc & 1;
+  for (int kc = 0; kc < QC; ++kc) {
+    const int c = QC * qc + kc, p = kc & 1;
     if (q == qc) colb[p][t] = r[kc];
     if (t == c) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) xb[p][16 * q + k] = x[k];
+      for (int k = 0; k < QC; ++k) xb[p][QC * q + k] = x[k];
     }
     __syncthreads();
-    double d = colb[p][c];
-    if (c < w) {
-      if (!(d > pm)) {
-        bad = true;
-        d = pm > 0.0 ? pm : 1.0;
-      }
-    } else {
-      d = 1.0;
+    // every LDS value of the step is read before any is used (no branches
+    // between the barrier and the loads: entries at or above the pivot are
+    // finite leftovers, masked by selects)
+    const double dv = colb[p][c], ct = colb[p][t];
+    double cl[QC], xr[QC];
+#pragma unroll
+    for (int k = 0; k < QC; ++k) {
+      cl[k] = colb[p][QC * q + k];
+      xr[k] = xb[p][QC * q + k];
     }
+    const bool ok = dv > pm;  // false for NaN
+    bad = bad || (c < w && !ok);
+    const double d = c < w ? (ok ? dv : (pm > 0.0 ? pm : 1.0)) : 1.0;
     const double inv = rsq_nr(d);
-    const double l = t > c ? colb[p][t] * inv : 0.0;
+    const double l = t > c ? ct * inv : 0.0;
     if (q == qc) {
       if (t == c) r[kc] = d * inv;
       if (t > c) r[kc] = l;
     }
     if (t == c) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] *= inv;
+      for (int k = 0; k < QC; ++k) x[k] *= inv;
     }
     const double li = l * inv;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const double lk = 16 * q + k > c ? colb[p][16 * q + k] : 0.0;
+    for (int k = 0; k < QC; ++k) {
+      const double lk = QC * q + k > c ? cl[k] : 0.0;
       r[k] -= li * lk;
-      x[k] -= li * xb[p][16 * q + k];
+      x[k] -= li * xr[k];
     }
   }
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int c = 16 * q + k;
+  for (int k = 0; k < QC; ++k) {
+    const int c = QC * q + k;
     if (t < w && c < w) F[(j0 + c) * ld + j0 + t] = c <= t ? r[k] : 0.0;
     X[c * QB + t] = (t < w && c < w) ? x[k] : 0.0;
   }
@@ -220,7 +225,7 @@ int cq_pass(double* S, double* F, int64_t m, int64_t n, double thr, const CqWs& 
   TN_HIP(hipMemsetAsync(F, 0, sizeof(double) * (size_t)(ld * n), st));
   for (int64_t j0 = 0; j0 < n; j0 += QB) {
     const int w = (int)std::min<int64_t>(QB, n - j0);
-    hipLaunchKernelGGL(cq_diag, dim3(1), dim3(256), 0, st, S, ld, j0, w, ws.pivmin, F, ws.X,
+    hipLaunchKernelGGL(cq_diag, dim3(1), dim3(512), 0, st, S, ld, j0, w, ws.pivmin, F, ws.X,
                        ws.fail);
     TN_CHECK_LAUNCH();
     // [L21; Q_j] = [G21; W_j] L11^{-T}:  B[k][c] = X[c][k] = X[k * QB + c]
