@@ -71,6 +71,10 @@ __global__ void __launch_bounds__(256) cq_pivmin(const double* __restrict__ S, i
 // replaced by *pivmin (or 1) so that everything stays finite; the caller
 // discards the result.  Padding rows/columns (>= w) are zero with unit
 // pivots.
+// DROP (the shifted PSD factorisation of tn_eigh_psd): *pivmin is the
+// shift delta; a pivot below delta / 2 (or not positive) drops its column
+// (L column and X row zero) instead of failing.
+template <bool DROP>
 __global__ void __launch_bounds__(512) cq_diag(const double* __restrict__ S, int64_t ld,
                                                int64_t j0, int w,
                                                const double* __restrict__ pivmin,
@@ -82,7 +86,7 @@ __global__ void __launch_bounds__(512) cq_diag(const double* __restrict__ S, int
   __shared__ double xb[2][QB];
   const int t = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
-  const double pm = *pivmin;
+  const double pm = DROP ? 0.5 * *pivmin : *pivmin;
   bool bad = false;
   double r[QC], x[QC];
 #pragma unroll
@@ -114,10 +118,17 @@ __global__ void __launch_bounds__(512) cq_diag(const double* __restrict__ S, int
       cl[k] = colb[p][QC * q + k];
       xr[k] = xb[p][QC * q + k];
     }
-    const bool ok = dv > pm;  // false for NaN
-    bad = bad || (c < w && !ok);
-    const double d = c < w ? (ok ? dv : (pm > 0.0 ? pm : 1.0)) : 1.0;
-    const double inv = rsq_nr(d);
+    double d, inv;
+    if (DROP) {
+      const bool keep = c >= w || (dv >= pm && dv > 0.0);
+      d = c < w ? (keep ? dv : 1.0) : 1.0;
+      inv = keep ? rsq_nr(d) : 0.0;
+    } else {
+      const bool ok = dv > pm;  // false for NaN
+      bad = bad || (c < w && !ok);
+      d = c < w ? (ok ? dv : (pm > 0.0 ? pm : 1.0)) : 1.0;
+      inv = rsq_nr(d);
+    }
     const double l = t > c ? ct * inv : 0.0;
     if (q == qc) {
       if (t == c) r[kc] = d * inv;
@@ -141,7 +152,7 @@ __global__ void __launch_bounds__(512) cq_diag(const double* __restrict__ S, int
     if (t < w && c < w) F[(j0 + c) * ld + j0 + t] = c <= t ? r[k] : 0.0;
     X[c * QB + t] = (t < w && c < w) ? x[k] : 0.0;
   }
-  if (bad && threadIdx.x == 0) fail[0] = 1;
+  if (!DROP && bad && threadIdx.x == 0) fail[0] = 1;
 }
 
 // orth = max |G - I| over the n x n block G (column-major, ld); block c
@@ -225,7 +236,7 @@ int cq_pass(double* S, double* F, int64_t m, int64_t n, double thr, const CqWs& 
   TN_HIP(hipMemsetAsync(F, 0, sizeof(double) * (size_t)(ld * n), st));
   for (int64_t j0 = 0; j0 < n; j0 += QB) {
     const int w = (int)std::min<int64_t>(QB, n - j0);
-    hipLaunchKernelGGL(cq_diag, dim3(1), dim3(512), 0, st, S, ld, j0, w, ws.pivmin, F, ws.X,
+    hipLaunchKernelGGL(cq_diag<false>, dim3(1), dim3(512), 0, st, S, ld, j0, w, ws.pivmin, F, ws.X,
                        ws.fail);
     TN_CHECK_LAUNCH();
     // [L21; Q_j] = [G21; W_j] L11^{-T}:  B[k][c] = X[c][k] = X[k * QB + c]
@@ -245,6 +256,14 @@ int cq_pass(double* S, double* F, int64_t m, int64_t n, double thr, const CqWs& 
 }
 
 }  // namespace
+
+int chol_block(const double* S, int64_t ld, int64_t j0, int w, const double* delta, double* F,
+               double* X, hipStream_t st) {
+  hipLaunchKernelGGL(cq_diag<true>, dim3(1), dim3(512), 0, st, S, ld, j0, w, delta, F, X,
+                     (int*)nullptr);
+  TN_CHECK_LAUNCH();
+  return TN_OK;
+}
 
 size_t cholqr_ws_bytes(int64_t m, int64_t n) {
   if (m <= 0 || n <= 0) return 256;

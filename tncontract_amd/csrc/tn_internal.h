@@ -35,6 +35,12 @@ int qr_form_q(const double* W, int64_t m, int64_t n, const QrWs& ws, double* Qw,
 // m >= n, FP64, decoded through rows/cols) = Q R, Q element (i, j) at
 // i * q_rs + j * q_cs, R = Q^T A row-major n x n.  Synchronises the stream;
 // TN_ERR_NOCONV when a pivot fails or max |Q^T Q - I| > orth_tol.
+// Shifted Cholesky of the w x w diagonal block at (j0, j0) of the
+// column-major S (ld): L11 into F at the same position, L11^{-1} into X
+// (64 x 64, column-major); a pivot below delta/2 (delta on the device)
+// drops its column (zero L column, zero X row).  w <= 64.
+int chol_block(const double* S, int64_t ld, int64_t j0, int w, const double* delta, double* F,
+               double* X, hipStream_t st);
 size_t cholqr_ws_bytes(int64_t m, int64_t n);
 int cholqr_run(const void* A, const Modes& rows, const Modes& cols, int64_t m, int64_t n,
                double* Q, int64_t q_rs, int64_t q_cs, double* R, double orth_tol,

@@ -522,9 +522,9 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
 //   1. P orders K by descending diagonal (a static pivot); shift
 //      delta = 8 sqrt(n) eps_in tr(K), eps_in the input's unit roundoff;
 //   2. P K P^T + delta I = L L^T, blocked right-looking Cholesky (64-column
-//      blocks: the diagonal block by one wavefront in registers, the panel
-//      by row-parallel forward substitution, the trailing update by an MFMA
-//      GEMM; a column whose pivot falls below delta/2 is dropped);
+//      blocks: the diagonal block and its inverse by one kernel, the panel
+//      L21 = A21 L11^{-T} and the trailing update by MFMA GEMMs; a column
+//      whose pivot falls below delta/2 is dropped);
 //   3. one-sided block Jacobi on the columns of L (no rotation accumulation):
 //      L Z = U' Sigma, hence P K P^T + delta I = U' Sigma^2 U'^T, so
 //      U = P^T U' and lam = sigma^2 - delta.
@@ -569,81 +569,6 @@ __global__ void psd_permute(const double* __restrict__ A, int64_t n, const int* 
   B[idx] = v;
 }
 
-// Factor the w x w diagonal block at (j0, j0) of B (column-major, ld n) and
-// write L11 (lower, zeros above) into Lw at the same position.  One
-// wavefront; lane t keeps row t of the block in registers (every loop has a
-// constant trip count, so the row is indexed statically); column c is
-// exchanged through LDS.  Padding columns (c >= w) are zero and stay zero.
-__global__ void __launch_bounds__(64) chol_diag(const double* __restrict__ B, int64_t n,
-                                                int64_t j0, int w,
-                                                const double* __restrict__ delta,
-                                                double* __restrict__ Lw) {
-  __shared__ double col[CB];
-  __shared__ double piv;
-  const int t = threadIdx.x;
-  const double dl = *delta;
-  double r[CB];
-#pragma unroll
-  for (int c = 0; c < CB; ++c) r[c] = (t < w && c < w) ? B[(j0 + c) * n + j0 + t] : 0.0;
-#pragma unroll
-  for (int c = 0; c < CB; ++c) {
-    if (t == c) piv = r[c];
-    __syncthreads();
-    // With the shift every exact pivot is >= delta; one below delta/2 means
-    // the input is indefinite beyond its rounding allowance: the column is
-    // dropped (zero) rather than divided by a tiny pivot.
-    const double d = piv;
-    const double sq = d >= 0.5 * dl && d > 0.0 ? sqrt(d) : 0.0;
-    const double inv = sq > 0.0 ? 1.0 / sq : 0.0;
-    if (t == c) r[c] = sq;
-    if (t > c) r[c] *= inv;
-    col[t] = t > c ? r[c] : 0.0;
-    __syncthreads();
-    if (t > c) {
-#pragma unroll
-      for (int c2 = c + 1; c2 < CB; ++c2) r[c2] -= r[c] * col[c2];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  if (t < w) {
-#pragma unroll
-    for (int c = 0; c < CB; ++c)
-      if (c < w) Lw[(j0 + c) * n + j0 + t] = t >= c ? r[c] : 0.0;
-  }
-}
-
-// Panel below the diagonal block: L21 = A21 L11^{-T} by forward
-// substitution, one row per lane (64 rows per block), rows and L11 in LDS;
-// each dot product runs in four independent partial sums.  Rows
-// [j0 + w, n) of columns [j0, j0 + w); A21 from B, L11 and L21 in Lw.
-__global__ void __launch_bounds__(64) chol_panel(const double* __restrict__ B, int64_t n,
-                                                 int64_t j0, int w, double* __restrict__ Lw) {
-  __shared__ double Ls[CB][CB + 1];
-  __shared__ double xs[CB][CB + 1];
-  __shared__ double rd[CB];
-  const int t = threadIdx.x;
-  for (int c = 0; c < CB; ++c) Ls[t][c] = (c < w && t < w) ? Lw[(j0 + c) * n + j0 + t] : 0.0;
-  const int64_t i = j0 + w + (int64_t)blockIdx.x * CB + t;
-  for (int c = 0; c < CB; ++c) xs[t][c] = (c < w && i < n) ? B[(j0 + c) * n + i] : 0.0;
-  __syncthreads();
-  rd[t] = Ls[t][t] > 0.0 ? 1.0 / Ls[t][t] : 0.0;
-  __syncthreads();
-  for (int c = 0; c < w; ++c) {
-    double a0 = xs[t][c], a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int j = 0;
-    for (; j + 3 < c; j += 4) {
-      a0 -= xs[t][j] * Ls[c][j];
-      a1 -= xs[t][j + 1] * Ls[c][j + 1];
-      a2 -= xs[t][j + 2] * Ls[c][j + 2];
-      a3 -= xs[t][j + 3] * Ls[c][j + 3];
-    }
-    for (; j < c; ++j) a0 -= xs[t][j] * Ls[c][j];
-    xs[t][c] = ((a0 + a1) + (a2 + a3)) * rd[c];
-  }
-  if (i < n)
-    for (int c = 0; c < w; ++c) Lw[(j0 + c) * n + i] = xs[t][c];
-}
-
 // U[p[i]][j] = Wv[perm[j]][i] / S[j];  lam[j] = S[j]^2 - delta
 __global__ void eig_out_kernel(const double* __restrict__ Wv, int64_t n, const int* __restrict__ p,
                                const int* __restrict__ perm, const double* __restrict__ S,
@@ -659,7 +584,7 @@ __global__ void eig_out_kernel(const double* __restrict__ Wv, int64_t n, const i
 
 struct EigWs {
   SvdWs jw;  // Jacobi buffers (Wv holds L)
-  double *B, *diag, *Sd, *delta;
+  double *B, *diag, *Sd, *delta, *X;
   int* p;
 };
 
@@ -674,6 +599,7 @@ static size_t eig_layout(int64_t n, char* base, size_t cap, EigWs* out) {
   e.diag = a.take<double>((size_t)n);
   e.Sd = a.take<double>((size_t)n);
   e.delta = a.take<double>(4);
+  e.X = a.take<double>((size_t)CB * CB);
   e.p = a.take<int>((size_t)n);
   e.jw.Gpart = a.take<double>((size_t)npairs * ceil_div(n, JCH) * J2 * J2);
   e.jw.sig = a.take<double>((size_t)n);
@@ -730,14 +656,14 @@ extern "C" int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes
   // 2. blocked Cholesky: B (working copy) -> L in Wv
   for (int64_t j0 = 0; j0 < n; j0 += CB) {
     const int w = (int)std::min<int64_t>(CB, n - j0);
-    hipLaunchKernelGGL(chol_diag, dim3(1), dim3(64), 0, st, e.B, n, j0, w, e.delta, Wv);
-    TN_CHECK_LAUNCH();
+    // L11 and its inverse X in one kernel (a dropped column has a zero X row)
+    TN_TRY(chol_block(e.B, n, j0, w, e.delta, Wv, e.X, st));
     const int64_t m2 = n - j0 - w;
     if (m2 <= 0) break;
-    // L21 = A21 L11^{-T}   (m2 x w)
-    hipLaunchKernelGGL(chol_panel, dim3((unsigned)ceil_div(m2, CB)), dim3(64), 0, st, e.B, n, j0, w,
-                       Wv);
-    TN_CHECK_LAUNCH();
+    // L21 = A21 L11^{-T} = A21 X^T   (m2 x w):  B[k][c] = X[c][k] = X[k * CB + c]
+    TN_TRY(contract_launch(e.B + j0 * n + j0 + w, mk({m2}, {1}), mk({w}, {n}), e.X, mk({w}, {CB}),
+                           mk({w}, {1}), Wv + j0 * n + j0 + w, mk({m2}, {1}), mk({w}, {n}), TN_F64,
+                           TN_PREC_NATIVE, 1.0, 0.0, nullptr, 0, st));
     // A22 -= L21 L21^T
     const double* L21 = Wv + j0 * n + j0 + w;
     TN_TRY(contract_launch(L21, mk({m2}, {1}), mk({w}, {n}), L21, mk({w}, {n}), mk({m2}, {1}),
