@@ -43,10 +43,14 @@ struct GemmArgs {
   void* part;      // split-K: dense M x N partial sums per z (nullptr: write C)
 };
 
-// Offset of index i in a mode group; single-mode groups avoid division.
+// Offset of index i in a mode group; single-mode groups avoid division and
+// indices below 2^32 (every k, and every row or column index in practice)
+// are decoded with 32-bit divisions, several times cheaper on the GPU than
+// 64-bit ones (the tile loads decode one k per element and k-tile).
 __device__ __forceinline__ int64_t goff(const Modes& m, int64_t i) {
   if (m.n == 1) return i * m.strides[0];
   if (m.n == 0) return 0;
+  if ((uint64_t)i <= 0xffffffffull) return mode_offset32(m, (uint32_t)i);
   return mode_offset(m, i);
 }
 
