@@ -78,6 +78,9 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
   __shared__ T Bs[2][BK * LDB];
   __shared__ int64_t rowA[SIMPLE ? 1 : BM], colB[SIMPLE ? 1 : BN], rowC[SIMPLE ? 1 : BM],
       colC[SIMPLE ? 1 : BN];
+  // multi-mode K groups: the BK offsets of a k-tile, decoded once per tile
+  // (by 2 BK threads, two tiles ahead) instead of once per element
+  __shared__ int64_t kA[SIMPLE ? 1 : 2][SIMPLE ? 1 : BK], kB[SIMPLE ? 1 : 2][SIMPLE ? 1 : BK];
 
   const T* __restrict__ A = (const T*)g.A;
   const T* __restrict__ B = (const T*)g.B;
@@ -91,7 +94,18 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
   const int64_t sbk = g.bk.n ? g.bk.strides[0] : 0, sbn = g.bn.n ? g.bn.strides[0] : 0;
   const int64_t scm = g.cm.n ? g.cm.strides[0] : 0, scn = g.cn.n ? g.cn.strides[0] : 0;
 
+  auto koffs = [&](int slot, int64_t k0) {
+    if (tid < BK) {
+      const int64_t k = k0 + tid;
+      kA[slot][tid] = k < kend ? goff(g.ak, k) : 0;
+    } else if (tid < 2 * BK) {
+      const int64_t k = k0 + tid - BK;
+      kB[slot][tid - BK] = k < kend ? goff(g.bk, k) : 0;
+    }
+  };
   if (!SIMPLE) {
+    koffs(0, kbeg);
+    koffs(1, kbeg + BK);
     for (int i = tid; i < BM; i += NT) {
       int64_t m = m0 + i;
       rowA[i] = m < g.M ? goff(g.am, m) : 0;
@@ -107,6 +121,7 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
 
   T ra[EPT_A], rb[EPT_B];
   auto load = [&](int64_t k0) {
+    const int slot = (int)(((k0 - kbeg) / BK) & 1);
 #pragma unroll
     for (int j = 0; j < EPT_A; ++j) {
       int e = tid + NT * j;
@@ -116,7 +131,7 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
       // SIMPLE operands are known (host check) to have every offset of the
       // tile, masked rows included, below 2^31: 32-bit index arithmetic
       const int64_t off = SIMPLE ? (int64_t)((uint32_t)m * (uint32_t)sam + (uint32_t)k * (uint32_t)sak)
-                                 : rowA[r] + goff(g.ak, k);
+                                 : rowA[r] + kA[slot][kk];
       ra[j] = (m < g.M && k < kend) ? A[off] : (T)0;
     }
 #pragma unroll
@@ -126,7 +141,7 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
       if (BNF >= 0 ? BNF == 1 : g.b_nfast != 0) { c = e % BN; kk = e / BN; } else { kk = e % BK; c = e / BK; }
       int64_t n = n0 + c, k = k0 + kk;
       const int64_t off = SIMPLE ? (int64_t)((uint32_t)k * (uint32_t)sbk + (uint32_t)n * (uint32_t)sbn)
-                                 : goff(g.bk, k) + colB[c];
+                                 : kB[slot][kk] + colB[c];
       rb[j] = (n < g.N && k < kend) ? B[off] : (T)0;
     }
   };
@@ -167,6 +182,8 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
   int cur = 0;
   for (int64_t kt = 0; kt < nk; ++kt) {
     if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+    // tile kt + 2 reuses the slot tile kt's loads read (one barrier ago)
+    if (!SIMPLE && kt + 2 < nk) koffs((int)(kt & 1), kbeg + (kt + 2) * BK);
     const T* as = As[cur];
     const T* bs = Bs[cur];
     // two k-steps of fragments in flight at a time (a full unroll hoists all
