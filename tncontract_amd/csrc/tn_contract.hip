@@ -120,8 +120,7 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
   }
 
   T ra[EPT_A], rb[EPT_B];
-  auto load = [&](int64_t k0) {
-    const int slot = (int)(((k0 - kbeg) / BK) & 1);
+  auto load = [&](int64_t k0, int slot) {
 #pragma unroll
     for (int j = 0; j < EPT_A; ++j) {
       int e = tid + NT * j;
@@ -175,13 +174,13 @@ __global__ void __launch_bounds__(NT, 2) gemm_mfma_kernel(GemmArgs g) {
 
   const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   if (nk > 0) {
-    load(kbeg);
+    load(kbeg, 0);
     store(0);
     __syncthreads();
   }
   int cur = 0;
   for (int64_t kt = 0; kt < nk; ++kt) {
-    if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+    if (kt + 1 < nk) load(kbeg + (kt + 1) * BK, (int)((kt + 1) & 1));
     // tile kt + 2 reuses the slot tile kt's loads read (one barrier ago)
     if (!SIMPLE && kt + 2 < nk) koffs((int)(kt & 1), kbeg + (kt + 2) * BK);
     const T* as = As[cur];
