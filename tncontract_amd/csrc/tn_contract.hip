@@ -251,6 +251,8 @@ __global__ void __launch_bounds__(NT) gemm_bf16x3_kernel(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) __bf16 Ah[BM * LD], Al[BM * LD];
   __shared__ __attribute__((aligned(16))) __bf16 Bh[BN * LD], Bl[BN * LD];
   __shared__ int64_t rowA[BM], colB[BN], rowC[BM], colC[BN];
+  // the BK offsets of a k-tile, decoded once per tile, two tiles ahead
+  __shared__ int64_t kA[2][BK], kB[2][BK];
 
   const float* __restrict__ A = (const float*)g.A;
   const float* __restrict__ B = (const float*)g.B;
@@ -259,6 +261,17 @@ __global__ void __launch_bounds__(NT) gemm_bf16x3_kernel(GemmArgs g) {
   const int64_t m0 = (int64_t)blockIdx.y * BM, n0 = (int64_t)blockIdx.x * BN;
   const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
   const int64_t kend = min(g.K, kbeg + g.kchunk);
+  auto koffs = [&](int slot, int64_t k0) {
+    if (tid < BK) {
+      const int64_t k = k0 + tid;
+      kA[slot][tid] = k < kend ? goff(g.ak, k) : 0;
+    } else if (tid < 2 * BK) {
+      const int64_t k = k0 + tid - BK;
+      kB[slot][tid - BK] = k < kend ? goff(g.bk, k) : 0;
+    }
+  };
+  koffs(0, kbeg);
+  koffs(1, kbeg + BK);
   for (int i = tid; i < BM; i += NT) {
     int64_t m = m0 + i;
     rowA[i] = m < g.M ? goff(g.am, m) : 0;
@@ -272,14 +285,14 @@ __global__ void __launch_bounds__(NT) gemm_bf16x3_kernel(GemmArgs g) {
   __syncthreads();
 
   float ra[EPT], rb[EPT];
-  auto load = [&](int64_t k0) {
+  auto load = [&](int64_t k0, int slot) {
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       int e = tid + NT * j;
       int r, kk;
       if (g.a_kfast) { kk = e % BK; r = e / BK; } else { r = e % BM; kk = e / BM; }
       int64_t m = m0 + r, k = k0 + kk;
-      ra[j] = (m < g.M && k < kend) ? A[rowA[r] + goff(g.ak, k)] : 0.f;
+      ra[j] = (m < g.M && k < kend) ? A[rowA[r] + kA[slot][kk]] : 0.f;
     }
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
@@ -287,7 +300,7 @@ __global__ void __launch_bounds__(NT) gemm_bf16x3_kernel(GemmArgs g) {
       int c, kk;
       if (g.b_nfast) { c = e % BN; kk = e / BN; } else { kk = e % BK; c = e / BK; }
       int64_t n = n0 + c, k = k0 + kk;
-      rb[j] = (n < g.N && k < kend) ? B[goff(g.bk, k) + colB[c]] : 0.f;
+      rb[j] = (n < g.N && k < kend) ? B[kB[slot][kk] + colB[c]] : 0.f;
     }
   };
   auto store = [&]() {
@@ -318,12 +331,14 @@ __global__ void __launch_bounds__(NT) gemm_bf16x3_kernel(GemmArgs g) {
 
   const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   if (nk > 0) {
-    load(kbeg);
+    load(kbeg, 0);
     store();
     __syncthreads();
   }
   for (int64_t kt = 0; kt < nk; ++kt) {
-    if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+    if (kt + 1 < nk) load(kbeg + (kt + 1) * BK, (int)((kt + 1) & 1));
+    // tile kt + 2 reuses the slot tile kt's loads read (before the last barrier)
+    if (kt + 2 < nk) koffs((int)(kt & 1), kbeg + (kt + 2) * BK);
     bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
