@@ -185,12 +185,38 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
   if (done[j]) return;
   const BJob J = jobs[j];
   const int pi = e_pair[item];
-  for (int e = tid; e < BJ2 * BJ2; e += 256) {
-    double acc = 0.0;
-    for (int c = 0; c < J.nch; ++c)
-      acc += Gpart[((int64_t)J.gram_base + (int64_t)pi * J.nch + c) * (BJ2 * BJ2) + e];
-    sm.G[0][e / BJ2][e % BJ2] = acc;
-    sm.Q[0][e / BJ2][e % BJ2] = (e / BJ2 == e % BJ2) ? 1.0 : 0.0;
+  {
+    // every partial of a group of four chunks loaded before the first add
+    constexpr int PER = BJ2 * BJ2 / 256;
+    const double* Gp = Gpart + ((int64_t)J.gram_base + (int64_t)pi * J.nch) * (BJ2 * BJ2);
+    double acc[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) acc[i] = 0.0;
+    int c = 0;
+    for (; c + 4 <= J.nch; c += 4) {
+      double v[4][PER];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int i = 0; i < PER; ++i) v[u][i] = Gp[(int64_t)(c + u) * (BJ2 * BJ2) + tid + 256 * i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int i = 0; i < PER; ++i) acc[i] += v[u][i];
+    }
+    for (; c < J.nch; ++c) {
+      double v[PER];
+#pragma unroll
+      for (int i = 0; i < PER; ++i) v[i] = Gp[(int64_t)c * (BJ2 * BJ2) + tid + 256 * i];
+#pragma unroll
+      for (int i = 0; i < PER; ++i) acc[i] += v[i];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + 256 * i;
+      sm.G[0][e / BJ2][e % BJ2] = acc[i];
+      sm.Q[0][e / BJ2][e % BJ2] = (e / BJ2 == e % BJ2) ? 1.0 : 0.0;
+    }
   }
   __syncthreads();
   const double off = eig_off(sm);

@@ -87,12 +87,38 @@ __device__ void gram_partial(double (*Ps)[64][J2 + 1], const double* __restrict_
 }
 
 // G (sm.G[0]) = sum of the pair's nchunk partials (fixed order, so every
-// block that sums them gets the same bits); Q (sm.Q[0]) = I.
+// block that sums them gets the same bits); Q (sm.Q[0]) = I.  The partials
+// are read four chunks at a time with every load issued before the first
+// add: a load-add chain would pay one memory latency per partial element.
 __device__ void eig_load(EigLds& sm, const double* __restrict__ Gp, int nchunk) {
-  for (int e = threadIdx.x; e < J2 * J2; e += 256) {
-    double acc = 0.0;
-    for (int c = 0; c < nchunk; ++c) acc += Gp[(int64_t)c * (J2 * J2) + e];
-    sm.G[0][e / J2][e % J2] = acc;
+  constexpr int PER = J2 * J2 / 256;  // elements per thread
+  double acc[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) acc[i] = 0.0;
+  int c = 0;
+  for (; c + 4 <= nchunk; c += 4) {
+    double v[4][PER];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int i = 0; i < PER; ++i)
+        v[u][i] = Gp[(int64_t)(c + u) * (J2 * J2) + threadIdx.x + 256 * i];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int i = 0; i < PER; ++i) acc[i] += v[u][i];
+  }
+  for (; c < nchunk; ++c) {
+    double v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) v[i] = Gp[(int64_t)c * (J2 * J2) + threadIdx.x + 256 * i];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) acc[i] += v[i];
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    sm.G[0][e / J2][e % J2] = acc[i];
     sm.Q[0][e / J2][e % J2] = (e / J2 == e % J2) ? 1.0 : 0.0;
   }
   lds_barrier();
