@@ -232,13 +232,23 @@ __device__ inline void apply_tile_finish(double* __restrict__ base, int64_t ld, 
 }
 
 // apply_tile_fetch + (Q from the row-major Qg in global memory into the LDS
-// buffer Q) + apply_tile_finish.
+// buffer Q) + apply_tile_finish.  Q's loads are issued first and all of
+// them before any LDS store, so storing Q waits only for Q (a load-store
+// loop would wait for the whole chunk fetch and then once per Q element).
 __device__ inline void apply_mfma(double* __restrict__ base, int64_t ld, int64_t nv, int64_t rs,
                                   int64_t re, int I, int J, const double* __restrict__ Qg,
                                   double (*Q)[J2 + 1], ApplyTile& Tw) {
+  constexpr int PER = J2 * J2 / 256;
+  double qv[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) qv[i] = Qg[threadIdx.x + 256 * i];
   double x[J2];
   apply_tile_fetch(base, ld, nv, rs, re, I, J, x);
-  for (int e = threadIdx.x; e < J2 * J2; e += 256) Q[e / J2][e % J2] = Qg[e];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    Q[e / J2][e % J2] = qv[i];
+  }
   apply_tile_finish(base, ld, nv, rs, re, I, J, x, Q, Tw);
 }
 
