@@ -146,11 +146,19 @@ __global__ void __launch_bounds__(256) bjac_gram(double* __restrict__ Wv, const 
   const double* base = Wv + J.wv;
   f64x4b acc = {0, 0, 0, 0};
   for (int64_t rb = r0; rb < r1; rb += 64) {
-    for (int e = tid; e < 64 * BJ2; e += 256) {
-      int row = e & 63, col = e >> 6;
-      int64_t v = col < BJB ? (int64_t)I * BJB + col : (int64_t)K2 * BJB + (col - BJB);
-      int64_t r = rb + row;
-      Ps[row][col] = (v < J.k && r < r1) ? base[v * J.L + r] : 0.0;
+    // the tile's eight loads per thread are all issued before the LDS stores
+    double pv[64 * BJ2 / 256];
+#pragma unroll
+    for (int i = 0; i < 64 * BJ2 / 256; ++i) {
+      const int e = tid + 256 * i, row = e & 63, col = e >> 6;
+      const int64_t v = col < BJB ? (int64_t)I * BJB + col : (int64_t)K2 * BJB + (col - BJB);
+      const int64_t r = rb + row;
+      pv[i] = (v < J.k && r < r1) ? base[v * J.L + r] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 64 * BJ2 / 256; ++i) {
+      const int e = tid + 256 * i;
+      Ps[e & 63][e >> 6] = pv[i];
     }
     __syncthreads();
 #pragma unroll
@@ -301,11 +309,19 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
   apply_tile_fetch(Z, J.k, J.k, 0, J.k, I, K2, z);
   f64x4b acc = {0, 0, 0, 0};
   for (int64_t rb = 0; rb < J.L; rb += 64) {
-    for (int e = tid; e < 64 * BJ2; e += 256) {
-      int row = e & 63, col = e >> 6;
-      int64_t v = col < BJB ? (int64_t)I * BJB + col : (int64_t)K2 * BJB + (col - BJB);
-      int64_t r = rb + row;
-      Ps[row][col] = (v < J.k && r < J.L) ? W[v * J.L + r] : 0.0;
+    // the tile's eight loads per thread are all issued before the LDS stores
+    double pv[64 * BJ2 / 256];
+#pragma unroll
+    for (int i = 0; i < 64 * BJ2 / 256; ++i) {
+      const int e = tid + 256 * i, row = e & 63, col = e >> 6;
+      const int64_t v = col < BJB ? (int64_t)I * BJB + col : (int64_t)K2 * BJB + (col - BJB);
+      const int64_t r = rb + row;
+      pv[i] = (v < J.k && r < J.L) ? W[v * J.L + r] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 64 * BJ2 / 256; ++i) {
+      const int e = tid + 256 * i;
+      Ps[e & 63][e >> 6] = pv[i];
     }
     __syncthreads();
 #pragma unroll
