@@ -43,7 +43,7 @@ NOTES = {"jac_eig_apply": "pair eigensolve (FP64 rotations, 31 dependent rounds 
 # HBM bytes per launch (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on a
 # streaming kernel) measured for the chi=512 sweep's n=1024 Jacobi launches;
 # profiles/jac_eig_chi512_pmc_traffic.md.  Used when --traffic is not given.
-MEASURED_TRAFFIC = {"jac_eig_apply": 15900303}
+MEASURED_TRAFFIC = {"jac_eig_apply": 17049600}
 
 
 def parse():

@@ -31,7 +31,8 @@
 
 namespace tn {
 
-constexpr int JCH = 256;     // rows per chunk (jac_gram / jac_eig_apply)
+constexpr int JCH = 256;     // rows per chunk of jac_eig_apply
+constexpr int GCH = 128;     // rows per Gram partial of jac_gram (twice the blocks)
 static_assert(JCH == 256, "apply_tile_load/finish cover 256 rows per block");
 
 // Partial Gram P^T P of the pair's 32 vectors over rows [r0, r1) into out
@@ -131,8 +132,8 @@ __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, i
   const int pi = blockIdx.x, ch = blockIdx.y;
   int I, J;
   pair_blocks(pi, s, nblk, I, J);
-  const int64_t r0 = (int64_t)ch * JCH;
-  const int64_t r1 = min(L, r0 + JCH);
+  const int64_t r0 = (int64_t)ch * GCH;
+  const int64_t r1 = min(L, r0 + GCH);
   gram_partial(Ps, Wv, L, nv, I, J, r0, r1, Gpart + ((int64_t)pi * nchunk + ch) * (J2 * J2));
 }
 
@@ -145,7 +146,7 @@ __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, i
 // through memory.
 __global__ void __launch_bounds__(256) jac_eig_apply(double* __restrict__ Wv, int64_t L,
                                                      double* __restrict__ Z, int64_t nv, int s,
-                                                     int nblk, int nchW,
+                                                     int nblk, int nchW, int nchG,
                                                      const double* __restrict__ Gpart, double tol,
                                                      unsigned long long* __restrict__ offmax) {
   __shared__ EigLds sm;
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(256) jac_eig_apply(double* __restrict__ Wv, in
   } else {
     base = Z; ld = nv; rs = (int64_t)(ch - nchW) * JCH; re = min(nv, rs + JCH);
   }
-  eig_load(sm, Gpart + (int64_t)pi * nchW * (J2 * J2), nchW);
+  eig_load(sm, Gpart + (int64_t)pi * nchG * (J2 * J2), nchG);
   double x[J2];
   apply_tile_fetch(base, ld, nv, rs, re, I, J, x);   // in flight during the rounds
   const double off = eig_off(sm);
@@ -303,7 +304,7 @@ static size_t svd_layout(int64_t m, int64_t n, char* base, size_t cap, SvdWs* ou
   Arena a{base, cap, 0};
   w.Wv = a.take<double>((size_t)(L * k));
   w.Z = a.take<double>((size_t)(k * k));
-  w.Gpart = a.take<double>((size_t)npairs * ceil_div(Lj, JCH) * J2 * J2);
+  w.Gpart = a.take<double>((size_t)npairs * ceil_div(Lj, GCH) * J2 * J2);
   w.sig = a.take<double>((size_t)k);
   w.S = a.take<double>((size_t)k);
   w.skip = a.take<int>((size_t)npairs);
@@ -332,6 +333,7 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
   const int npairs = nblk / 2;
   // Z == nullptr: the rotations are not accumulated (eigensolver use)
   const int nchW = (int)ceil_div(L, JCH), nchZ = Z ? (int)ceil_div(k, JCH) : 0;
+  const int nchG = (int)ceil_div(L, GCH);
   const int64_t kz = Z ? k : 0;
   if (Z)
     hipLaunchKernelGGL(init_identity, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, st, Z, k);
@@ -347,8 +349,8 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
     for (int s = 0; s < nblk - 1; ++s) {
       {
         ProfScope p(PROF_JAC_GRAM, st, gram_fl, 8.0 * (double)L * J2 * npairs);
-        hipLaunchKernelGGL(jac_gram, dim3(npairs, nchW), dim3(256), 0, st, Wv, L, k, s, nblk,
-                           nchW, w.Gpart);
+        hipLaunchKernelGGL(jac_gram, dim3(npairs, nchG), dim3(256), 0, st, Wv, L, k, s, nblk,
+                           nchG, w.Gpart);
       }
       {
         // a kernel boundary between the partials and the pair eigensolve is
@@ -356,10 +358,10 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
         // kernel (measured).  Work: the inner sweep (31 rounds x 9216 flops)
         // once per pair plus the apply; bytes: partials per block + chunks
         ProfScope p(PROF_JAC_EIG, st, 31.0 * 9216.0 * npairs + apply_fl,
-                    8.0 * (double)nchW * J2 * J2 * npairs * (nchW + nchZ) +
+                    8.0 * (double)nchG * J2 * J2 * npairs * (nchW + nchZ) +
                         16.0 * (double)(L + kz) * J2 * npairs);
         hipLaunchKernelGGL(jac_eig_apply, dim3(npairs, nchW + nchZ), dim3(256), 0, st, Wv, L, Z,
-                           k, s, nblk, nchW, w.Gpart, tol, w.off);
+                           k, s, nblk, nchW, nchG, w.Gpart, tol, w.off);
       }
     }
     TN_CHECK_LAUNCH();
@@ -627,7 +629,7 @@ static size_t eig_layout(int64_t n, char* base, size_t cap, EigWs* out) {
   e.delta = a.take<double>(4);
   e.X = a.take<double>((size_t)CB * CB);
   e.p = a.take<int>((size_t)n);
-  e.jw.Gpart = a.take<double>((size_t)npairs * ceil_div(n, JCH) * J2 * J2);
+  e.jw.Gpart = a.take<double>((size_t)npairs * ceil_div(n, GCH) * J2 * J2);
   e.jw.sig = a.take<double>((size_t)n);
   e.jw.S = a.take<double>((size_t)n);
   e.jw.skip = a.take<int>((size_t)npairs);
