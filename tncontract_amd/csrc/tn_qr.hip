@@ -233,10 +233,20 @@ __global__ void __launch_bounds__(256) qr_panel_dots(const double* __restrict__ 
   const int64_t r0 = c0 + (int64_t)blockIdx.y * QCH;
   const int64_t cbase = cb + (int64_t)blockIdx.x * QCOL;
   const int tid = threadIdx.x;
-  for (int e = tid; e < QCH * W; e += 256) {
-    int rr = e / W, a = e % W;
-    int64_t r = r0 + rr;
-    Vs[rr][a] = (a < w && r < m) ? vref(A, ld, c0, a, r) : 0.0;
+  {
+    // all of a thread's reflector loads issued before its LDS stores
+    double vv[QCH * W / 256];
+#pragma unroll
+    for (int i = 0; i < QCH * W / 256; ++i) {
+      const int e = tid + 256 * i, rr = e / W, a = e % W;
+      const int64_t r = r0 + rr;
+      vv[i] = (a < w && r < m) ? vref(A, ld, c0, a, r) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < QCH * W / 256; ++i) {
+      const int e = tid + 256 * i;
+      Vs[e / W][e % W] = vv[i];
+    }
   }
   __syncthreads();
   double y[W][QCOL];
@@ -280,22 +290,41 @@ __global__ void __launch_bounds__(256) qr_panel_update(double* __restrict__ A, i
                                                        int64_t ncol, int nchunk,
                                                        const double* __restrict__ Ypart,
                                                        const double* __restrict__ Tp) {
+  __shared__ double ys[W][QCOL];
   __shared__ double z[W][QCOL];
   const int64_t r0 = c0 + (int64_t)blockIdx.y * QCH;
   const int64_t cbase = cb + (int64_t)blockIdx.x * QCOL;
   const int tid = threadIdx.x;
   if (tid < W * QCOL) {
-    // z[b][j] = sum_{a <= b} T[a][b] y[a][j]
-    int b = tid / QCOL, j = tid % QCOL;
-    int64_t c = cbase + j;
-    double acc = 0.0;
-    if (b < w && c < cb + ncol) {
-      for (int a = 0; a <= b; ++a) {
-        double ya = 0.0;
-        for (int ch = 0; ch < nchunk; ++ch) ya += Ypart[((int64_t)ch * W + a) * ncol + (c - cb)];
-        acc += Tp[a * w + b] * ya;
+    // y[a][j] = sum over the chunks' partials (in chunk order), four loads
+    // in flight at a time
+    const int a = tid / QCOL, j = tid % QCOL;
+    const int64_t c = cbase + j;
+    double ya = 0.0;
+    if (a < w && c < cb + ncol) {
+      const double* yp = Ypart + (int64_t)a * ncol + (c - cb);
+      const int64_t cs = (int64_t)W * ncol;
+      int ch = 0;
+      for (; ch + 4 <= nchunk; ch += 4) {
+        const double v0 = yp[ch * cs], v1 = yp[(ch + 1) * cs];
+        const double v2 = yp[(ch + 2) * cs], v3 = yp[(ch + 3) * cs];
+        ya += v0;
+        ya += v1;
+        ya += v2;
+        ya += v3;
       }
+      for (; ch < nchunk; ++ch) ya += yp[ch * cs];
     }
+    ys[a][j] = ya;
+  }
+  __syncthreads();
+  if (tid < W * QCOL) {
+    // z[b][j] = sum_{a <= b} T[a][b] y[a][j]
+    const int b = tid / QCOL, j = tid % QCOL;
+    const int64_t c = cbase + j;
+    double acc = 0.0;
+    if (b < w && c < cb + ncol)
+      for (int a = 0; a <= b; ++a) acc += Tp[a * w + b] * ys[a][j];
     z[b][j] = acc;
   }
   __syncthreads();
