@@ -563,7 +563,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (every box here has one GPU); "
+        print("bench.py: measures one GPU (the sweep does not shard within a chain); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == "batched":
