@@ -120,6 +120,46 @@ struct BJob {
   double tol;
 };
 
+// acc += the 32 x 32 Gram of the pair's vectors (blocks I and K2 of the
+// job's k vectors of length L at W) over rows [r0, r1): 64-row tiles staged
+// through LDS, v_mfma_f64_16x16x4 by four waves (one 16 x 16 quarter each);
+// the next tile's loads are issued into registers before the current
+// tile's MFMAs, so their latency overlaps the MFMAs.
+__device__ __forceinline__ void bgram_rows(const double* __restrict__ W, int64_t L, int k, int I,
+                                           int K2, int64_t r0, int64_t r1,
+                                           double (*Ps)[BJ2 + 1], f64x4b& acc) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ti = wave >> 1, tj = wave & 1;
+  constexpr int PE = 64 * BJ2 / 256;
+  double pv[PE];
+  auto fetch = [&](int64_t rb) {
+#pragma unroll
+    for (int i = 0; i < PE; ++i) {
+      const int e = tid + 256 * i, row = e & 63, col = e >> 6;
+      const int64_t v = col < BJB ? (int64_t)I * BJB + col : (int64_t)K2 * BJB + (col - BJB);
+      const int64_t r = rb + row;
+      pv[i] = (v < k && r < r1) ? W[v * L + r] : 0.0;
+    }
+  };
+  if (r0 < r1) fetch(r0);
+  for (int64_t rb = r0; rb < r1; rb += 64) {
+#pragma unroll
+    for (int i = 0; i < PE; ++i) {
+      const int e = tid + 256 * i;
+      Ps[e & 63][e >> 6] = pv[i];
+    }
+    __syncthreads();
+    if (rb + 64 < r1) fetch(rb + 64);
+#pragma unroll
+    for (int k0 = 0; k0 < 64; k0 += 4) {
+      double a = Ps[k0 + (lane >> 4)][16 * ti + (lane & 15)];
+      double b = Ps[k0 + (lane >> 4)][16 * tj + (lane & 15)];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+}
+
 __device__ __forceinline__ int brr(int pos, int step, int nblk) {
   return pos == 0 ? 0 : 1 + (pos - 1 + step) % (nblk - 1);
 }
@@ -145,30 +185,7 @@ __global__ void __launch_bounds__(256) bjac_gram(double* __restrict__ Wv, const 
   if (work && tid == 0) atomicAdd(work, (unsigned long long)(r1 - r0));
   const double* base = Wv + J.wv;
   f64x4b acc = {0, 0, 0, 0};
-  for (int64_t rb = r0; rb < r1; rb += 64) {
-    // the tile's eight loads per thread are all issued before the LDS stores
-    double pv[64 * BJ2 / 256];
-#pragma unroll
-    for (int i = 0; i < 64 * BJ2 / 256; ++i) {
-      const int e = tid + 256 * i, row = e & 63, col = e >> 6;
-      const int64_t v = col < BJB ? (int64_t)I * BJB + col : (int64_t)K2 * BJB + (col - BJB);
-      const int64_t r = rb + row;
-      pv[i] = (v < J.k && r < r1) ? base[v * J.L + r] : 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < 64 * BJ2 / 256; ++i) {
-      const int e = tid + 256 * i;
-      Ps[e & 63][e >> 6] = pv[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k0 = 0; k0 < 64; k0 += 4) {
-      double a = Ps[k0 + (lane >> 4)][16 * ti + (lane & 15)];
-      double b = Ps[k0 + (lane >> 4)][16 * tj + (lane & 15)];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
-    __syncthreads();
-  }
+  bgram_rows(base, J.L, J.k, I, K2, r0, r1, Ps, acc);
   double* out = Gpart + (int64_t)item * (BJ2 * BJ2);
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -308,30 +325,7 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
   apply_tile_fetch(W, J.L, J.k, 0, J.L, I, K2, x);  // in flight during the Gram and rounds
   apply_tile_fetch(Z, J.k, J.k, 0, J.k, I, K2, z);
   f64x4b acc = {0, 0, 0, 0};
-  for (int64_t rb = 0; rb < J.L; rb += 64) {
-    // the tile's eight loads per thread are all issued before the LDS stores
-    double pv[64 * BJ2 / 256];
-#pragma unroll
-    for (int i = 0; i < 64 * BJ2 / 256; ++i) {
-      const int e = tid + 256 * i, row = e & 63, col = e >> 6;
-      const int64_t v = col < BJB ? (int64_t)I * BJB + col : (int64_t)K2 * BJB + (col - BJB);
-      const int64_t r = rb + row;
-      pv[i] = (v < J.k && r < J.L) ? W[v * J.L + r] : 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < 64 * BJ2 / 256; ++i) {
-      const int e = tid + 256 * i;
-      Ps[e & 63][e >> 6] = pv[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k0 = 0; k0 < 64; k0 += 4) {
-      double a = Ps[k0 + (lane >> 4)][16 * ti + (lane & 15)];
-      double b = Ps[k0 + (lane >> 4)][16 * tj + (lane & 15)];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
-    __syncthreads();
-  }
+  bgram_rows(W, J.L, J.k, I, K2, 0, J.L, Ps, acc);
 #pragma unroll
   for (int e = 0; e < 4; ++e) sm.G[0][16 * ti + (lane >> 4) + 4 * e][16 * tj + (lane & 15)] = acc[e];
   for (int e = tid; e < BJ2 * BJ2; e += 256) sm.Q[0][e / BJ2][e % BJ2] = (e / BJ2 == e % BJ2) ? 1.0 : 0.0;
