@@ -156,8 +156,11 @@ int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int
 /* ---- SVD (np.linalg.svd(full_matrices=False), tensor.py:915) ------------
  * Thin SVD of the m x n matrix A (decoded as in tn_qr): U m x k row-major,
  * S k (FP64, descending), Vt k x n row-major, k = min(m,n).  One-sided
- * block Jacobi with FP64 rotations.  tol: orthogonality tolerance (<=0 ->
- * max(m,n) * eps, as LAPACK dgesvj); max_sweeps <= 0 -> 40.  *sweeps_out (host, may be NULL) receives
+ * block Jacobi with FP64 rotations; with k >= 32 the vectors are first
+ * ordered by norm and QR-factored (Cholesky QR, Householder when that does
+ * not reach orthogonality 1e-13) and the Jacobi runs on the rows of R.
+ * tol: orthogonality tolerance (<= 0 -> (vector length) * eps, as LAPACK
+ * dgesvj); max_sweeps <= 0 -> 40.  *sweeps_out (host, may be NULL) receives
  * the number of sweeps used.  Returns TN_ERR_NOCONV if not converged. */
 size_t tn_svd_workspace(int64_t m, int64_t n, int dtype);
 int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m, int64_t n,
