@@ -89,6 +89,12 @@ size_t tn_sumsq_workspace(int64_t n);
 int tn_sumsq(const void* x, int dtype, int64_t n, double* out, void* ws, size_t ws_bytes,
              void* stream);
 
+/* out[0] (device) = number of infinite entries of x, out[1] = number of NaN
+ * entries: the diagnostic tensor_svd prints before re-raising when an SVD
+ * fails (tensor.py:922-929).  One workgroup; for error paths only. */
+int tn_count_nonfinite(const void* x, int dtype, int64_t n, unsigned long long* out,
+                       void* stream);
+
 /* x /= ||x||_F in place, ||x|| computed in FP64 and stored at *norm_out
  * (device double, may be NULL); x untouched when the norm is 0.  Keeps the
  * left environments of the structured sweep O(1) without a host round trip. */
@@ -190,13 +196,17 @@ int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes* k_cols, i
  * No single reference call: each job is what the reference does with one
  * tensordot (tensor.py:735) followed by truncated_svd (tensor.py:1112) with a
  * relative threshold -- here for many independent jobs of different sizes in
- * shared launches (grouped FP32-MFMA GEMM with autotuned tile shapes, then a
- * batched one-sided block Jacobi with FP64 rotations).
+ * shared launches (grouped GEMM of the FP32 factors on FP64 MFMA with FP64
+ * accumulation and autotuned tile shapes, then a batched one-sided block
+ * Jacobi with FP64 rotations).
  * C_j = L_j R_j (m x n); C_j = U S V^T (thin, sorted descending).  kept_out[j]
- * (host) = #{sigma/sigma_0 > rel_threshold}, capped at chi_max (>0).  The
- * first kept columns of U / rows of Vt are the truncated factors.  The Jacobi
- * stops at orthogonality max(1e-10, L eps) (L the vector length): C_j is
- * formed in FP32, so its singular vectors are defined to ~1e-7 anyway. */
+ * (host) = #{sigma/sigma_0 > rel_threshold}, capped at chi_max (>0): C_j is
+ * FP64-accurate, so this is the rank of the exact product.  The first kept
+ * columns of U / rows of Vt are the truncated factors.  The Jacobi stops at
+ * orthogonality L eps (L the vector length); vectors whose squared norm is
+ * below (rel_threshold / 2k)^2 ||C_j||_F^2 are not rotated against each
+ * other (their span lies below the threshold), so the singular values and
+ * vectors past the kept ones are not those of an SVD. */
 typedef struct tn_trunc_job {
   const void* L; /* m x k FP32 row-major */
   const void* R; /* k x n FP32 row-major */

@@ -33,6 +33,13 @@ How each function is pinned (checks live in ``tests/test_oracle.py``):
   * variational_compress: exact at sufficient chi, never worse than its
     svd_compress start, left canonical, and the projection identities
     <w|v> = <w|w>, |v - w|^2 = |v|^2 - |w|^2 of the final single-site update.
+  * tensor_to_matrix / matrix_to_tensor: element-by-element known answers
+    with explicit index loops, and their round trip.
+  * the keep rule of left_canonise's SVD path: two-site states with known
+    Schmidt values scaled so that a relative and an absolute threshold keep
+    different ranks; the result equals the dense Eckart-Young truncation.
+  * init_mps_random: each site equals its np.random.rand draw divided by that
+    draw's largest singular value (np.linalg.norm(., 2)).
   * reference fixture tests/golden/random_10site_mps.npz (extracted from the
     reference's tests/random_10site_mps.dat without unpickling): the
     invariants of tncontract/tests/test_canonical_form_mps_conversion.py.
@@ -52,7 +59,7 @@ __all__ = [
     "tensor_qr", "OMPS", "OMPO", "contract_mps_mpo", "ladder_contract",
     "inner_product_mps", "frob_distance_squared", "expvals_mps",
     "svd_compress_mps", "left_canonical_form_mps", "right_canonical_form_mps",
-    "unique_label",
+    "unique_label", "init_mps_random",
 ]
 
 
@@ -357,7 +364,20 @@ def tensor_svd(tensor, row_labels, svd_label="svd_", absorb_singular_values=None
     col_labels = [x for x in t.labels if x not in row_labels]
     old_shape = t.data.shape
     mat = np.reshape(t.data, (rows, t.data.size // rows))
-    u, s, v = np.linalg.svd(mat, full_matrices=False)
+    try:
+        u, s, v = np.linalg.svd(mat, full_matrices=False)
+    except (np.linalg.LinAlgError, ValueError):
+        # tensor.py:916-929: retry with LAPACK gesvd; non-finite input raises
+        # ValueError after printing what it holds
+        import warnings
+        import scipy.linalg
+        warnings.warn('numpy.linalg.svd failed, trying scipy.linalg.svd with lapack_driver="gesvd"')
+        try:
+            u, s, v = scipy.linalg.svd(mat, full_matrices=False, lapack_driver="gesvd")
+        except ValueError:
+            print("tensor_svd failed. Matrix contains inf's: " + str(np.isinf(mat).any())
+                  + ". Matrix contains nan's: " + str(np.isnan(mat).any()))
+            raise
     U = OTensor(np.reshape(u, list(old_shape[:len(row_labels)]) + [u.shape[1]]),
                 list(row_labels) + [svd_label + "in"])
     V = OTensor(np.reshape(v, [v.shape[0]] + list(old_shape[len(row_labels):])),
@@ -928,6 +948,27 @@ def variational_compress_mps(mps, chi, max_iter=10, initial_guess=None, toleranc
     """onedim_core.py:1474-1478"""
     return mps.variational_compress(chi, max_iter=max_iter, initial_guess=initial_guess,
                                     tolerance=tolerance)
+
+
+def init_mps_random(nsites, physdim, bonddim=1, left_label="left", right_label="right",
+                    phys_label="phys"):
+    """onedim_utils.py:23-60: np.random.rand sites, each divided through its
+    SVD over [phys, left] | right so that its largest singular value is 1."""
+    if not np.iterable(physdim):
+        physdim = [physdim] * nsites
+    if not np.iterable(bonddim):
+        bonddim = [bonddim] * (nsites - 1)
+    bonddim = [1] + list(bonddim) + [1]
+    tensors = []
+    for i in range(nsites):
+        rt = OTensor(np.random.rand(physdim[i], bonddim[i], bonddim[i + 1]),
+                     [phys_label, left_label, right_label])
+        U, S, V = tensor_svd(rt, [phys_label, left_label])
+        S.data = S.data / S.data[0, 0]
+        rt = contract(U, S, ["svd_in"], ["svd_out"])
+        rt = contract(rt, V, ["svd_in"], ["svd_out"])
+        tensors.append(rt)
+    return OMPS(tensors, left_label=left_label, right_label=right_label, phys_label=phys_label)
 
 
 def conjugate(tensor):

@@ -238,3 +238,79 @@ def test_tall_qr_and_svd_beyond_register_panel(tn, m, n):
     np.testing.assert_allclose(np.diag(S.numpy()), s_ref, rtol=1e-11)
     rec = U.numpy() @ S.numpy() @ V.numpy()
     np.testing.assert_allclose(rec, a, atol=1e-11 * np.abs(a).max())
+
+
+def test_tensor_to_matrix_and_back(tn):
+    # tensor.py:805-827: bit-exact data movement, same shapes and labels
+    rng = np.random.default_rng(21)
+    a = rng.standard_normal((2, 3, 4, 5))
+    g, r = both(tn, a, ["a", "b", "c", "d"])
+    for rows in (["c", "a"], [], ["d", "c", "b", "a"], ["b"]):
+        Mg = tn.tensor_to_matrix(g, rows)
+        Mr = o.tensor_to_matrix(r, rows)
+        assert tuple(Mg.shape) == Mr.shape
+        assert np.array_equal(Mg.cpu().numpy(), Mr)
+    assert g.labels == ["a", "b", "c", "d"]
+    m = rng.standard_normal((6, 20))
+    tg = tn.matrix_to_tensor(m, (2, 3, 4, 5))
+    tr = o.matrix_to_tensor(m, (2, 3, 4, 5))
+    assert tg.labels == tr.labels == ["i0", "i1", "i2", "i3"]
+    assert np.array_equal(tg.numpy(), tr.data)
+    tg = tn.matrix_to_tensor(tn.tensor_to_matrix(g, ["c", "a"]), (4, 2, 3, 5), ["c", "a", "b", "d"])
+    assert tg.labels == ["c", "a", "b", "d"]
+    assert np.array_equal(tg.numpy(), np.transpose(a, (2, 0, 1, 3)))
+
+
+@pytest.mark.parametrize("scale,threshold,rel_keep", [(1e-3, 1e-10, 3), (1e3, 1e-14, 3)])
+def test_left_canonise_threshold_is_relative_gpu(tn, scale, threshold, rel_keep):
+    # onedim_core.py:333-339: sigma / sigma_0 > threshold; an absolute rule
+    # would keep 2 (scale 1e-3) or 4 (scale 1e3) of these Schmidt values
+    from cases import schmidt_pair
+    sigma = scale * np.array([1.0, 1e-3, 1e-9, 1e-16])
+    arrs, U, V, _ = schmidt_pair(sigma, 8, 5)
+    g, r = gpu_mps(tn, arrs), o_mps(arrs)
+    g.left_canonise(threshold=threshold)
+    r.left_canonise(threshold=threshold)
+    assert g.bonddims() == r.bonddims() == [1, rel_keep, 1]
+    expect = ((U[:, :rel_keep] * sigma[:rel_keep]) @ V[:, :rel_keep].T).reshape(-1)
+    assert state_err(to_dense_gpu(g), expect) < 1e-20
+    assert state_err(to_dense_gpu(g), om.mps_to_dense(r)) < 1e-20
+
+
+def test_init_mps_random_matches_oracle(tn):
+    # onedim_utils.py:23-60 with the same np.random stream
+    np.random.seed(3)
+    g = tn.onedim.init_mps_random(6, 2, [4, 5, 5, 3, 2])
+    np.random.seed(3)
+    r = o.init_mps_random(6, 2, [4, 5, 5, 3, 2])
+    assert g.bonddims() == r.bonddims() == [1, 4, 5, 5, 3, 2, 1]
+    for a, b in zip(g, r):
+        assert a.labels == b.labels
+        np.testing.assert_allclose(a.numpy(), b.data, rtol=1e-12, atol=1e-14)
+        s = np.linalg.svd(a.numpy().reshape(-1, a.shape[2]), compute_uv=False)
+        assert abs(s[0] - 1.0) < 1e-12
+
+
+def test_svd_nonconvergence_is_retried(tn):
+    # tensor.py:914-921: a failed SVD is retried (there with gesvd) after a
+    # warning.  Forced from the host with a one-sweep limit.
+    from tncontract_amd import tensor as tsr
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((60, 40))
+    with pytest.warns(UserWarning, match="did not converge"):
+        u, s, vt = tsr._svd_device(torch.from_numpy(a).cuda(), 1, max_sweeps=1)
+    np.testing.assert_allclose(s.cpu().numpy(), np.linalg.svd(a, compute_uv=False), rtol=1e-12)
+    rec = (u.cpu().numpy() * s.cpu().numpy()) @ vt.cpu().numpy()
+    np.testing.assert_allclose(rec, a, atol=1e-12 * np.abs(a).max())
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_svd_of_nonfinite_matrix_raises_like_the_reference(tn, capsys, bad):
+    # tensor.py:922-929: the diagnostic line, then ValueError
+    a = np.ones((6, 5))
+    a[2, 3] = bad
+    with pytest.raises(ValueError):
+        tn.tensor_svd(tn.Tensor(a, ["r", "c"]), ["r"])
+    out = capsys.readouterr().out
+    assert ("Matrix contains inf's: %s. Matrix contains nan's: %s"
+            % (bool(np.isinf(bad)), bool(np.isnan(bad)))) in out

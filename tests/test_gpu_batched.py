@@ -1,8 +1,9 @@
 """Batched uneven contraction + truncation (grouped GEMM + batched Jacobi)
-against NumPy FP64 on the same FP32 inputs.  Inputs are FP32 and the
-contraction accumulates in FP32, so singular values are compared to 1e-4
-relative above 1e-5 * sigma_0 and the kept rank with a threshold (1e-4) that
-lies well above the FP32 noise floor."""
+against NumPy FP64 (the oracle's tensordot + LAPACK SVD) on the same FP32
+factors.  The product is accumulated in FP64, so the kept rank at the
+config's relative threshold 1e-8 equals LAPACK's, the kept singular values
+agree to 1e-12 sigma_0 and the truncated factors reconstruct LAPACK's
+truncation to 1e-10 ||C||."""
 import numpy as np
 import pytest
 
@@ -19,26 +20,54 @@ def tn():
     return tncontract_amd
 
 
-@pytest.mark.parametrize("njobs,lo,hi", [(6, 16, 40), (40, 16, 160)])
-def test_batched_truncate_matches_numpy(tn, njobs, lo, hi):
+def check_job(Lm, Rm, U, S, Vt, kept, thr, chi):
+    C = Lm.astype(np.float64) @ Rm.astype(np.float64)
+    u_ref, s_ref, vt_ref = np.linalg.svd(C, full_matrices=False)
+    k_ref = int(np.count_nonzero(s_ref / s_ref[0] > thr))
+    if chi:
+        k_ref = min(k_ref, chi)
+    assert kept == k_ref, (kept, k_ref)
+    s = S.cpu().numpy()
+    np.testing.assert_allclose(s[:kept], s_ref[:kept], rtol=0, atol=1e-12 * s_ref[0])
+    u = U.cpu().numpy().astype(np.float64)[:, :kept]
+    vt = Vt.cpu().numpy().astype(np.float64)[:kept]
+    rec = (u * s[:kept]) @ vt
+    ref = (u_ref[:, :kept] * s_ref[:kept]) @ vt_ref[:kept]
+    assert np.linalg.norm(rec - ref) <= 1e-10 * np.linalg.norm(C)
+    np.testing.assert_allclose(u.T @ u, np.eye(kept), atol=1e-10)
+    np.testing.assert_allclose(vt @ vt.T, np.eye(kept), atol=1e-10)
+
+
+@pytest.mark.parametrize("njobs,lo,hi,chi", [(6, 16, 40, 0), (40, 16, 160, 64), (40, 16, 160, 0)])
+def test_batched_truncate_matches_numpy(tn, njobs, lo, hi, chi):
     from tncontract_amd.batched import batched_truncate, config3_jobs
-    jobs = config3_jobs(njobs, seed=njobs, chi_lo=lo, chi_hi=hi)
+    jobs = config3_jobs(njobs, seed=njobs + chi, chi_lo=lo, chi_hi=hi)
     Ls = [torch.from_numpy(j[0]).cuda() for j in jobs]
     Rs = [torch.from_numpy(j[1]).cuda() for j in jobs]
-    thr, chi = 1e-4, 64
-    U, S, Vt, kept, sweeps = batched_truncate(Ls, Rs, rel_threshold=thr, chi=chi)
+    U, S, Vt, kept, sweeps = batched_truncate(Ls, Rs, rel_threshold=1e-8, chi=chi)
     for j, (Lm, Rm, sig) in enumerate(jobs):
-        C = Lm.astype(np.float64) @ Rm.astype(np.float64)
-        s_ref = np.linalg.svd(C, compute_uv=False)
-        s = S[j].cpu().numpy()
-        big = s_ref > 1e-5 * s_ref[0]
-        np.testing.assert_allclose(s[:big.sum()], s_ref[big], rtol=1e-4, atol=1e-6 * s_ref[0])
-        k_ref = min(int(np.count_nonzero(s_ref / s_ref[0] > thr)), chi)
-        assert kept[j] == k_ref, (j, kept[j], k_ref)
-        u, vt = U[j].cpu().numpy().astype(np.float64), Vt[j].cpu().numpy().astype(np.float64)
-        k = kept[j]
-        rec = (u[:, :k] * s[:k]) @ vt[:k]
-        ref = (np.linalg.svd(C)[0][:, :k] * s_ref[:k]) @ np.linalg.svd(C)[2][:k]
-        assert np.linalg.norm(rec - ref) <= 1e-4 * np.linalg.norm(C)
-        np.testing.assert_allclose(u[:, :k].T @ u[:, :k], np.eye(k), atol=1e-5)
+        check_job(Lm, Rm, U[j], S[j], Vt[j], int(kept[j]), 1e-8, chi)
     assert sweeps.max() < 40
+
+
+def test_config3_full_size(tn):
+    """Config 3 at its stated size: 1024 jobs, chi_l, chi_m, chi_r ~
+    U{16..512}, d=2, sigma_k = exp(-k/tau), tau ~ U[4, 64], FP32 factors,
+    keep sigma/sigma_0 > 1e-8.  Every job's kept rank is compared with the
+    closed form min(rank, #{exp(-k/tau) > 1e-8}) up to the FP32 rounding of
+    the factors; 48 jobs, spread over the sizes, against LAPACK in full."""
+    from tncontract_amd.batched import batched_truncate, config3_jobs
+    jobs = config3_jobs(1024, seed=0)
+    Ls = [torch.from_numpy(j[0]).cuda() for j in jobs]
+    Rs = [torch.from_numpy(j[1]).cuda() for j in jobs]
+    U, S, Vt, kept, sweeps = batched_truncate(Ls, Rs, rel_threshold=1e-8)
+    assert sweeps.max() < 40
+    for j, (Lm, Rm, sig) in enumerate(jobs):
+        # rounding the factors to FP32 moves each sigma by ~1e-7 sigma_0
+        lo = int(np.count_nonzero(sig > 1e-8 + 1e-6))
+        hi = int(np.count_nonzero(sig > 1e-8 - 1e-6))
+        assert lo <= kept[j] <= hi, (j, kept[j], lo, hi)
+    order = np.argsort([j[0].shape[0] * j[1].shape[1] for j in jobs])
+    for j in order[::len(order) // 48][:48]:
+        Lm, Rm, _ = jobs[j]
+        check_job(Lm, Rm, U[j], S[j], Vt[j], int(kept[j]), 1e-8, 0)

@@ -1,11 +1,15 @@
 """The structured apply+compress sweep reproduces contract_mps_mpo +
 svd_compress (oracle) on the GPU: same bond dimensions and labels, state
-error <= 1e-10 in FP64 and <= 1e-8 with FP32 / bf16x3 contractions."""
+error <= 1e-10 in FP64 and <= 1e-8 with FP32 / bf16x3 contractions.  A bond
+whose rank the sweep cannot resolve (decided below its Gram resolution floor)
+sends the call to the exact path, whose result is held to <= 1e-10."""
 import numpy as np
 import pytest
 
 from oracle import tn_oracle as o
 from oracle import models as om
+
+from cases import decaying_sigma, identity_mpo_arrays, schmidt_chain
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -51,6 +55,7 @@ def test_structured_matches_oracle(tn, N, chi, chi_out, prec, tol):
     err = state_error(g, r)
     assert abs(err) < tol, err
     assert g.check_canonical_form(threshold=1e-10, print_output=False)[1] == 0
+    assert st.fallback is None          # random spectra: every chi cut is resolved
 
 
 def test_structured_matches_plain_gpu_path(tn):
@@ -65,3 +70,85 @@ def test_structured_matches_plain_gpu_path(tn):
     n = tn.onedim.inner_product_mps(plain, plain)
     err = (tn.onedim.inner_product_mps(fast, fast) + n - 2 * tn.onedim.inner_product_mps(fast, plain)) / n
     assert abs(float(err)) < 1e-10
+
+
+def _identity_mpo(tn, N, d):
+    return tn.onedim.MatrixProductOperator(
+        [tn.Tensor(w, ["left", "right", "physout", "physin"]) for w in identity_mpo_arrays(N, d)],
+        "left", "right", "physout", "physin")
+
+
+def _oracle_compress(arrs, mpo_arrs, chi, thr):
+    r = o.OMPS([o.OTensor(a, ["phys", "left", "right"]) for a in arrs])
+    w = o.OMPO([o.OTensor(x, ["left", "right", "physout", "physin"]) for x in mpo_arrs],
+               "left", "right", "physout", "physin")
+    r = o.contract_mps_mpo(r, w)
+    r.svd_compress(chi=chi, threshold=thr)
+    return r
+
+
+# A 4-site chain with d = 8 whose middle bond (dimension 64) carries the
+# Schmidt values sigma_k = exp(-k/4), k < 64 (1 ... 1.4e-7); the outer bonds
+# (dimension 8) are never cut.  Identity MPO, so apply+compress is
+# svd_compress of that state.  chi = 12 / 28 / 55 cut at sigma/sigma_0 ~
+# 5e-2 / 1e-3 / 1.2e-6; chi = None with threshold 2e-6 keeps 53 values.
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("chi,thr", [(12, 1e-15), (28, 1e-15), (55, 1e-15), (None, 2e-6)])
+def test_structured_decaying_spectrum(tn, prec, chi, thr):
+    from tncontract_amd.onedim.structured import apply_mpo_compress, SweepStats
+    N, d = 4, 8
+    sigma = decaying_sigma(64, 4.0)
+    arrs = schmidt_chain(N, d, sigma, 7)
+    psi = tn.onedim.MatrixProductState([tn.Tensor(a, ["phys", "left", "right"]) for a in arrs])
+    st = SweepStats()
+    g = apply_mpo_compress(psi, _identity_mpo(tn, N, d), chi=chi, threshold=thr, precision=prec,
+                           stats=st)
+    r = _oracle_compress(arrs, identity_mpo_arrays(N, d), chi, thr)
+    assert g.bonddims() == r.bonddims()
+    assert r.bonddims()[2] == (chi if chi else 53)
+    assert [t.labels for t in g] == [t.labels for t in r]
+    err = state_error(g, r)
+    exact = prec == "fp64" or st.fallback is not None
+    assert abs(err) < (1e-10 if exact else 1e-8), (err, st.fallback, st.floor)
+    # which path ran: the cut at 5e-2 is above every floor; the cut at 1e-3
+    # and the threshold 2e-6 are above the FP64 floor (~4e-7 here) and below
+    # the FP32 one (~1e-2)
+    if chi == 12 or (chi in (28, None) and prec == "fp64"):
+        assert st.fallback is None, st.floor
+    if chi in (28, None) and prec != "fp64":
+        assert st.fallback is not None, st.floor
+
+
+def test_structured_rank_deficient_product_state(tn):
+    # a product state times the XXZ MPO: every bond of the result has a few
+    # exactly-zero Schmidt values; with chi=None the rank is threshold-bound
+    # below the Gram resolution, so the exact path decides (same bond
+    # dimensions as the reference, not the Gram's rounding-noise count)
+    from tncontract_amd.onedim import xxz_mpo
+    from tncontract_amd.onedim.structured import apply_mpo_compress, SweepStats
+    N = 8
+    rng = np.random.default_rng(4)
+    arrs = [rng.standard_normal((2, 1, 1)) for _ in range(N)]
+    psi = tn.onedim.MatrixProductState([tn.Tensor(a, ["phys", "left", "right"]) for a in arrs])
+    st = SweepStats()
+    g = apply_mpo_compress(psi, xxz_mpo(N), chi=None, threshold=1e-15, precision="fp64", stats=st)
+    r = oracle_result(arrs, N, None, 1e-15)
+    assert g.bonddims() == r.bonddims()
+    assert abs(state_error(g, r)) < 1e-10
+    assert st.fallback is not None
+
+
+def test_structured_zero_state_raises_like_the_reference(tn):
+    # contract_mps_mpo + svd_compress of a zero state: the reference divides
+    # by the zero norm and its SVD of the NaN site raises ValueError
+    from tncontract_amd.onedim import xxz_mpo
+    from tncontract_amd.onedim.structured import apply_mpo_compress
+    arrs = [np.zeros((2, 1 if i == 0 else 3, 1 if i == 3 else 3)) for i in range(4)]
+    psi = tn.onedim.MatrixProductState([tn.Tensor(a, ["phys", "left", "right"]) for a in arrs])
+    with pytest.warns(Warning):
+        with pytest.raises(ValueError):
+            apply_mpo_compress(psi, xxz_mpo(4), chi=2, precision="fp32")
+    plain = tn.onedim.MatrixProductState([tn.Tensor(a, ["phys", "left", "right"]) for a in arrs])
+    with pytest.warns(Warning):
+        with pytest.raises(ValueError):
+            plain.svd_compress(chi=2)

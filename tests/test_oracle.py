@@ -12,6 +12,8 @@ import pytest
 from oracle import tn_oracle as o
 from oracle import models as om
 
+from cases import decaying_sigma, schmidt_chain, schmidt_pair
+
 
 def T(shape, labels, rng):
     return o.OTensor(rng.standard_normal(shape), labels)
@@ -396,3 +398,101 @@ def test_variational_compress_target_unchanged_and_nonconvergence(rng):
     np.testing.assert_allclose(om.mps_to_dense(tgt), v, atol=1e-12 * np.linalg.norm(v))
     with pytest.raises(RuntimeError):
         tgt.variational_compress(2, max_iter=1, tolerance=1e-30)
+
+
+# ---------------------------------------------------------------- tensor <-> matrix
+def test_tensor_to_matrix_known_answer(rng):
+    # tensor.py:805-818: the row labels are moved to the front in the given
+    # order (one move_index each), the rest keep their relative order; the
+    # row index runs fastest over the LAST row label.  Checked element by
+    # element with explicit index loops (no reshape).
+    t = T((2, 3, 4, 5), ["a", "b", "c", "d"], rng)
+    M = o.tensor_to_matrix(t, ["c", "a"])
+    assert M.shape == (8, 15)
+    for a in range(2):
+        for b in range(3):
+            for c in range(4):
+                for d in range(5):
+                    assert M[c * 2 + a, b * 5 + d] == t.data[a, b, c, d]
+    # the input tensor is not modified
+    assert t.labels == ["a", "b", "c", "d"] and t.shape == (2, 3, 4, 5)
+    # no row labels: one row; every label a row label: one column
+    assert o.tensor_to_matrix(t, []).shape == (1, 120)
+    M2 = o.tensor_to_matrix(t, ["d", "c", "b", "a"])
+    assert M2.shape == (120, 1)
+    assert M2[((4 * 4 + 3) * 3 + 2) * 2 + 1, 0] == t.data[1, 2, 3, 4]
+
+
+def test_matrix_to_tensor_known_answer(rng):
+    # tensor.py:821-827: row-major reshape, default labels i0, i1, ...
+    m = rng.standard_normal((6, 4))
+    t = o.matrix_to_tensor(m, (2, 3, 4))
+    assert t.labels == ["i0", "i1", "i2"] and t.shape == (2, 3, 4)
+    for i in range(2):
+        for j in range(3):
+            for k in range(4):
+                f = (i * 3 + j) * 4 + k           # row-major flat position
+                assert t.data[i, j, k] == m[f // 4, f % 4]
+    t2 = o.matrix_to_tensor(m, (3, 8), ["x", "y"])
+    assert t2.labels == ["x", "y"] and t2.data[2, 7] == m[5, 3]
+    # round trip with tensor_to_matrix
+    back = o.tensor_to_matrix(o.matrix_to_tensor(m, (2, 3, 4), ["p", "q", "r"]), ["p", "q"])
+    assert np.array_equal(back, m)
+
+
+# ---------------------------------------------------------------- keep rule of left_canonise
+@pytest.mark.parametrize("scale,threshold,rel_keep,abs_keep", [
+    (1e-3, 1e-10, 3, 2),      # relative keeps more than an absolute rule would
+    (1e3, 1e-14, 3, 4),       # relative keeps fewer than an absolute rule would
+])
+def test_left_canonise_threshold_is_relative(scale, threshold, rel_keep, abs_keep):
+    # onedim_core.py:333-339: sigma / sigma_0 > threshold (sigma_0 the largest
+    # value at that bond), then [:chi]; the removed sigma_0 factors return at
+    # the end of the segment (:357-358).  An absolute rule keeps abs_keep.
+    sigma = scale * np.array([1.0, 1e-3, 1e-9, 1e-16])
+    assert np.count_nonzero(sigma > threshold) == abs_keep
+    arrs, U, V, _ = schmidt_pair(sigma, 8, 5)
+    m = o.OMPS([o.OTensor(a, ["phys", "left", "right"]) for a in arrs])
+    m.left_canonise(threshold=threshold)
+    assert m.bonddims() == [1, rel_keep, 1]
+    expect = (U[:, :rel_keep] * sigma[:rel_keep]) @ V[:, :rel_keep].T
+    got = om.mps_to_dense(m).reshape(8, 8)
+    np.testing.assert_allclose(got, expect, atol=1e-13 * scale)
+    # chi caps after the threshold
+    m = o.OMPS([o.OTensor(a, ["phys", "left", "right"]) for a in arrs])
+    m.left_canonise(threshold=threshold, chi=2)
+    assert m.bonddims() == [1, 2, 1]
+
+
+def test_init_mps_random_closed_form():
+    # onedim_utils.py:23-60: site i = rand draw / its largest singular value
+    np.random.seed(11)
+    m = o.init_mps_random(5, 2, [3, 4, 4, 2])
+    np.random.seed(11)
+    dims = [1, 3, 4, 4, 2, 1]
+    for i, t in enumerate(m):
+        raw = np.random.rand(2, dims[i], dims[i + 1])
+        expect = raw / np.linalg.norm(raw.reshape(2 * dims[i], dims[i + 1]), 2)
+        assert t.labels == ["phys", "left", "right"]
+        np.testing.assert_allclose(t.data, expect, rtol=1e-12, atol=1e-14)
+    assert m.bonddims() == dims
+
+
+def test_schmidt_chain_has_the_given_spectrum():
+    # the builder of the decaying-spectrum tests: the middle bond's Schmidt
+    # values of the dense state are exactly sigma
+    sigma = decaying_sigma(16, 3.0)
+    arrs = schmidt_chain(10, 2, sigma, 1)
+    m = o.OMPS([o.OTensor(a, ["phys", "left", "right"]) for a in arrs])
+    v = om.mps_to_dense(m).reshape(2 ** 5, 2 ** 5)
+    np.testing.assert_allclose(np.linalg.svd(v, compute_uv=False)[:16], sigma, rtol=1e-12)
+
+
+def test_svd_compress_of_a_zero_state_raises_like_the_reference():
+    # onedim_core.py:476-480 divides by the zero norm; the right sweep's SVD of
+    # the NaN site then fails in both LAPACK drivers (tensor.py:914-929)
+    arrs = [np.zeros((2, 1 if i == 0 else 3, 1 if i == 3 else 3)) for i in range(4)]
+    m = o.OMPS([o.OTensor(a, ["phys", "left", "right"]) for a in arrs])
+    with pytest.warns(Warning):
+        with pytest.raises(ValueError):
+            m.svd_compress(chi=2)

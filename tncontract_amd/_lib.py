@@ -78,6 +78,7 @@ def lib():
         "tn_diag_inverse": (i32, [vp, i64, i64, i64, vp, i32, vp, vp, sz, vp]),
         "tn_sumsq_workspace": (sz, [i64]),
         "tn_sumsq": (i32, [vp, i32, i64, vp, vp, sz, vp]),
+        "tn_count_nonfinite": (i32, [vp, i32, i64, vp, vp]),
         "tn_trace": (i32, [vp, mp, i64, i64, vp, i32, vp]),
         "tn_normalize_workspace": (sz, [i64]),
         "tn_normalize": (i32, [vp, i32, i64, vp, vp, sz, vp]),
@@ -269,6 +270,15 @@ def sumsq(t):
     ws = workspace(lib().tn_sumsq_workspace(n))
     check(lib().tn_sumsq(ptr(src), dtype_code(src), n, ptr(out), ptr(ws), ws.numel(), stream_ptr()))
     return float(out.item())
+
+
+def count_nonfinite(t):
+    """(number of infinite entries, number of NaN entries) of t (host ints)."""
+    src = contiguous(t)
+    out = torch.empty(2, dtype=torch.int64, device=t.device)
+    check(lib().tn_count_nonfinite(ptr(src), dtype_code(src), src.numel(), ptr(out), stream_ptr()))
+    h = out.cpu().numpy()
+    return int(h[0]), int(h[1])
 
 
 def normalize_(t, norm_out=None):

@@ -3,10 +3,13 @@
 ``batched_truncate(lefts, rights, rel_threshold, chi)`` computes, for every
 job j, C_j = L_j @ R_j and its truncated SVD (keep sigma/sigma_0 >
 rel_threshold, at most chi), all jobs in shared launches of
-libtncontract_amd (grouped FP32-MFMA GEMM with autotuned tile shapes,
-batched one-sided Jacobi with FP64 rotations).  Per job this is what the
-reference computes with ``contract`` (tensor.py:635) followed by
-``truncated_svd(..., absolute=False)`` (tensor.py:1112).
+libtncontract_amd (grouped GEMM of the FP32 factors with FP64 accumulation
+and autotuned tile shapes, batched one-sided Jacobi with FP64 rotations,
+replayed as a captured HIP graph).  Per job this is what the reference
+computes with ``contract`` (tensor.py:635) followed by
+``truncated_svd(..., absolute=False)`` (tensor.py:1112); the product is
+FP64-accurate, so the kept rank is the one the reference's FP64 tensordot of
+the same factors gives.
 """
 from __future__ import annotations
 

@@ -4,16 +4,25 @@
 // tensordot (tensor.py:735) + one tensor_svd/truncated_svd (tensor.py:830,
 // 1112).  Everything below runs every job in shared launches:
 //
-//  * grouped GEMM (FP32 MFMA, v_mfma_f32_16x16x4): one launch covers the
-//    tiles of all jobs (block -> (job, tile) by binary search over a tile
-//    prefix); the epilogue writes each C_j as FP64 directly into the Jacobi
-//    vector layout (columns of C_j if tall, rows if wide).  Two tile shapes
-//    (64x64, 128x128); a small autotuner times both per size bucket on first
-//    use and caches the choice.
+//  * grouped GEMM (FP32 factors, FP64 MFMA v_mfma_f64_16x16x4 with FP64
+//    accumulation, so the rank decided at a relative threshold is the exact
+//    product's): one launch covers the tiles of all jobs (block -> (job,
+//    tile) by binary search over a tile prefix); the epilogue writes each C_j
+//    directly into the Jacobi vector layout (columns of C_j if tall, rows if
+//    wide).  Two tile shapes (64x64, 128x128); a small autotuner times both
+//    per size bucket on first use and caches the choice.
 //  * batched one-sided block Jacobi: every job advances its own round-robin
 //    schedule (step g mod (nblk_j - 1)) in the same gram / eig / apply
 //    launches; a per-job sweep-end kernel retires converged jobs on the
-//    device; the host polls the number of active jobs every few steps.
+//    device.  A set of steps (at least one sweep of the largest job) is
+//    captured once as a HIP graph and replayed, the device advancing its own
+//    step counter; the host reads each set's active-job count while the next
+//    set runs and stops replaying when none is left.
+//    Vectors whose squared norm is below (thr / 2k)^2 ||C_j||_F^2 are not
+//    rotated against each other (bjob_neg2): the rounding-level vectors of a
+//    rank-deficient product lie below the keep threshold whatever their
+//    orthogonality, and orthogonalising that cluster was most of the sweeps
+//    of such jobs.
 //  * per-job sort, rank selection (sigma/sigma_0 > threshold, <= chi) and
 //    output of U, S, V^T.
 #include "tn_internal.h"
@@ -26,6 +35,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace tn {
@@ -42,15 +52,22 @@ struct GJob {
   int64_t crs, ccs;
 };
 
-template <int FR>  // MFMA fragments per wave along each dimension (tile = 32*FR)
-__global__ void __launch_bounds__(256) gemm_grouped_f32(const GJob* __restrict__ jobs,
-                                                       const int* __restrict__ tile_base,
-                                                       const int* __restrict__ job_ids, int njobs) {
+// FP64 = true: the FP32 operands are converted to FP64 (exactly) in the LDS
+// store and multiplied by v_mfma_f64_16x16x4, so C carries FP64 rounding
+// (~1e-16 relative) instead of FP32's (~1e-7): the rank a relative threshold
+// of 1e-8 selects is then the rank of the exact product, as in the
+// reference's FP64 tensordot of the same data.  FP64 = false: FP32 MFMA.
+template <int FR, bool FP64>  // FR: MFMA fragments per wave along each dimension (tile 32*FR)
+__global__ void __launch_bounds__(256) gemm_grouped(const GJob* __restrict__ jobs,
+                                                   const int* __restrict__ tile_base,
+                                                   const int* __restrict__ job_ids, int njobs) {
+  typedef typename std::conditional<FP64, double, float>::type T;
+  typedef typename std::conditional<FP64, f64x4b, f32x4b>::type V4;
   constexpr int BT = 32 * FR;  // block tile (square)
   constexpr int BK = 16;
   constexpr int LD = BT + 2;
-  __shared__ float As[BK * LD];
-  __shared__ float Bs[BK * LD];
+  __shared__ T As[BK * LD];
+  __shared__ T Bs[BK * LD];
   // locate job: largest j with tile_base[j] <= blockIdx.x
   int lo = 0, hi = njobs - 1;
   const int b = blockIdx.x;
@@ -65,25 +82,25 @@ __global__ void __launch_bounds__(256) gemm_grouped_f32(const GJob* __restrict__
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 1) * (16 * FR), wc = (wave & 1) * (16 * FR);
   const int lr = lane & 15, lk = lane >> 4;
-  f32x4b acc[FR][FR];
+  V4 acc[FR][FR];
 #pragma unroll
   for (int i = 0; i < FR; ++i)
 #pragma unroll
-    for (int j = 0; j < FR; ++j) acc[i][j] = f32x4b{0, 0, 0, 0};
+    for (int j = 0; j < FR; ++j) acc[i][j] = V4{0, 0, 0, 0};
   for (int k0 = 0; k0 < g.K; k0 += BK) {
     // A tile: lanes walk k (unit stride); B tile: lanes walk n (unit stride)
     for (int e = tid; e < BT * BK; e += 256) {
       int kk = e % BK, r = e / BK;
       int m = m0 + r, k = k0 + kk;
-      As[kk * LD + r] = (m < g.M && k < g.K) ? g.A[(int64_t)m * g.K + k] : 0.f;
+      As[kk * LD + r] = (T)((m < g.M && k < g.K) ? g.A[(int64_t)m * g.K + k] : 0.f);
       int c = e % BT, kb = e / BT;
       int n = n0 + c, k2 = k0 + kb;
-      Bs[kb * LD + c] = (n < g.N && k2 < g.K) ? g.B[(int64_t)k2 * g.N + n] : 0.f;
+      Bs[kb * LD + c] = (T)((n < g.N && k2 < g.K) ? g.B[(int64_t)k2 * g.N + n] : 0.f);
     }
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
-      float af[FR], bf[FR];
+      T af[FR], bf[FR];
 #pragma unroll
       for (int i = 0; i < FR; ++i) af[i] = As[(ks + lk) * LD + wr + 16 * i + lr];
 #pragma unroll
@@ -91,18 +108,24 @@ __global__ void __launch_bounds__(256) gemm_grouped_f32(const GJob* __restrict__
 #pragma unroll
       for (int i = 0; i < FR; ++i)
 #pragma unroll
-        for (int j = 0; j < FR; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FR; ++j) {
+          if constexpr (FP64)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
     }
     __syncthreads();
   }
+  // accumulator element e of lane (lr, lk): row 4 lk + e (FP32) or lk + 4 e
+  // (FP64) of the 16 x 16 fragment, column lr
 #pragma unroll
   for (int i = 0; i < FR; ++i)
 #pragma unroll
     for (int j = 0; j < FR; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        int r = m0 + wr + 16 * i + 4 * lk + e, c = n0 + wc + 16 * j + lr;
+        int r = m0 + wr + 16 * i + (FP64 ? lk + 4 * e : 4 * lk + e), c = n0 + wc + 16 * j + lr;
         if (r < g.M && c < g.N) g.C[(int64_t)r * g.crs + (int64_t)c * g.ccs] = (double)acc[i][j][e];
       }
 }
@@ -118,6 +141,9 @@ struct BJob {
   int32_t L, k, nblk, nch, nchz;
   int32_t gram_base, eig_base;
   double tol;
+  // squared norm below which a vector is negligible (pairs of two such
+  // vectors are not rotated; 0: none is)
+  double neg2;
 };
 
 // acc += the 32 x 32 Gram of the pair's vectors (blocks I and K2 of the
@@ -168,7 +194,8 @@ __global__ void __launch_bounds__(256) bjac_gram(double* __restrict__ Wv, const 
                                                  const int* __restrict__ it_job,
                                                  const int* __restrict__ it_pair,
                                                  const int* __restrict__ it_chunk,
-                                                 const int* __restrict__ done, int g,
+                                                 const int* __restrict__ done,
+                                                 const int* __restrict__ gbase, int goff,
                                                  double* __restrict__ Gpart,
                                                  unsigned long long* __restrict__ work) {
   __shared__ double Ps[64][BJ2 + 1];
@@ -176,7 +203,7 @@ __global__ void __launch_bounds__(256) bjac_gram(double* __restrict__ Wv, const 
   const int j = it_job[item];
   if (done[j]) return;
   const BJob J = jobs[j];
-  const int s = g % (J.nblk - 1);
+  const int s = (*gbase + goff) % (J.nblk - 1);
   const int pi = it_pair[item], ch = it_chunk[item];
   const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -244,7 +271,7 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
     }
   }
   __syncthreads();
-  const double off = eig_off(sm);
+  const double off = eig_off(sm, J.neg2);
   if (tid == 0) atomicMax(offmax + j, dbits(off));
   if (off <= J.tol) {
     if (tid == 0) skip[item] = 1;
@@ -252,7 +279,7 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
   }
   if (tid == 0) skip[item] = 0;
   if (work && tid == 0) atomicAdd(work + 1, 1ull);
-  const int cur = eig_rotate(sm, J.tol);
+  const int cur = eig_rotate(sm, J.tol, J.neg2);
   double* out = Qall + (int64_t)item * (BJ2 * BJ2);
   for (int e = tid; e < BJ2 * BJ2; e += 256) out[e] = sm.Q[cur][e / BJ2][e % BJ2];
 }
@@ -263,7 +290,8 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
                                                   const int* __restrict__ a_job,
                                                   const int* __restrict__ a_pair,
                                                   const int* __restrict__ a_chunk,
-                                                  const int* __restrict__ done, int g,
+                                                  const int* __restrict__ done,
+                                                  const int* __restrict__ gbase, int goff,
                                                   const double* __restrict__ Qall,
                                                   const int* __restrict__ skip,
                                                   unsigned long long* __restrict__ work) {
@@ -276,7 +304,7 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
   const int pi = a_pair[item], ch = a_chunk[item];
   const int eidx = J.eig_base + pi;
   if (skip[eidx]) return;
-  const int s = g % (J.nblk - 1);
+  const int s = (*gbase + goff) % (J.nblk - 1);
   const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
   const double* Qg = Qall + (int64_t)eidx * BJ2 * BJ2;
   if (work && threadIdx.x == 0) {
@@ -304,7 +332,8 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
                                                   const BJob* __restrict__ jobs,
                                                   const int* __restrict__ e_job,
                                                   const int* __restrict__ e_pair,
-                                                  const int* __restrict__ done, int g,
+                                                  const int* __restrict__ done,
+                                                  const int* __restrict__ gbase, int goff,
                                                   unsigned long long* __restrict__ offmax,
                                                   unsigned long long* __restrict__ work) {
   __shared__ double Ps[64][BJ2 + 1];
@@ -314,7 +343,7 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
   const int j = e_job[item];
   if (done[j]) return;
   const BJob J = jobs[j];
-  const int s = g % (J.nblk - 1);
+  const int s = (*gbase + goff) % (J.nblk - 1);
   const int pi = e_pair[item];
   const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
   const int wave = tid >> 6, lane = tid & 63;
@@ -330,7 +359,7 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
   for (int e = 0; e < 4; ++e) sm.G[0][16 * ti + (lane >> 4) + 4 * e][16 * tj + (lane & 15)] = acc[e];
   for (int e = tid; e < BJ2 * BJ2; e += 256) sm.Q[0][e / BJ2][e % BJ2] = (e / BJ2 == e % BJ2) ? 1.0 : 0.0;
   __syncthreads();
-  const double off = eig_off(sm);
+  const double off = eig_off(sm, J.neg2);
   if (tid == 0) atomicMax(offmax + j, dbits(off));
   if (off <= J.tol) return;
   if (work && tid == 0) {
@@ -338,19 +367,20 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
     atomicAdd(work + 1, 1ull);
     atomicAdd(work + 2, (unsigned long long)(J.L + J.k));
   }
-  const int cur = eig_rotate(sm, J.tol);
+  const int cur = eig_rotate(sm, J.tol, J.neg2);
   apply_tile_finish(W, J.L, J.k, 0, J.L, I, K2, x, sm.Q[cur], Tw);
   apply_tile_finish(Z, J.k, J.k, 0, J.k, I, K2, z, sm.Q[cur], Tw);
 }
 
 // After global step g: retire jobs whose sweep just ended with off <= tol.
-__global__ void bjac_sweep_end(const BJob* __restrict__ jobs, int njobs, int g, int max_sweeps,
+__global__ void bjac_sweep_end(const BJob* __restrict__ jobs, int njobs,
+                               const int* __restrict__ gbase, int goff, int max_sweeps,
                                int* __restrict__ done, int* __restrict__ sweeps,
                                unsigned long long* __restrict__ offmax, int* __restrict__ active) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= njobs || done[j]) return;
   const BJob J = jobs[j];
-  if ((g + 1) % (J.nblk - 1) != 0) return;
+  if ((*gbase + goff + 1) % (J.nblk - 1) != 0) return;
   double off = __longlong_as_double((long long)offmax[j]);
   offmax[j] = 0ull;
   int sw = ++sweeps[j];
@@ -363,11 +393,43 @@ __global__ void bjac_sweep_end(const BJob* __restrict__ jobs, int njobs, int g, 
   }
 }
 
+// The global step of a launch is *gbase + goff: a captured set of steps
+// (goff = 0 .. n-1) is replayed with the base advanced by n after each set.
+__global__ void badvance(int* __restrict__ gbase, int n) {
+  if (threadIdx.x == 0) *gbase += n;
+}
+
 __global__ void binit_identity(double* __restrict__ Zb, const BJob* __restrict__ jobs) {
   const BJob J = jobs[blockIdx.y];
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)J.k * J.k) return;
   Zb[J.z + idx] = (idx / J.k == idx % J.k) ? 1.0 : 0.0;
+}
+
+// neg2 of every job from its Frobenius norm F (the sum of its vectors'
+// squared norms, which the rotations keep): with every vector of squared norm
+// <= neg2 = (thr / (2 k))^2 F^2 left unrotated against the others like it,
+// the space those vectors span has singular values <= sqrt(k neg2) <=
+// thr F / (2 sqrt(k)) <= thr sigma_0 / 2 (F <= sqrt(k) sigma_0), so it lies
+// below the relative threshold thr whatever its internal orthogonality.
+__global__ void __launch_bounds__(256) bjob_neg2(const double* __restrict__ Wv,
+                                                 BJob* __restrict__ jobs, double thr) {
+  __shared__ double red[4];
+  BJob& J = jobs[blockIdx.x];
+  const int64_t tot = (int64_t)J.L * J.k;
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < tot; e += 256) {
+    const double x = Wv[J.wv + e];
+    acc = fma(x, x, acc);
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double f2 = red[0] + red[1] + red[2] + red[3];
+    const double q = thr / (2.0 * (double)J.k);
+    J.neg2 = q * q * f2;
+  }
 }
 
 __global__ void __launch_bounds__(256) bvec_norms(const double* __restrict__ Wv,
@@ -514,10 +576,10 @@ static int launch_grouped(int FR, const std::vector<int>& ids, const std::vector
   TN_HIP(hipStreamSynchronize(st));  // host tables are locals
   if (tot == 0) return TN_OK;
   if (FR == 2)
-    hipLaunchKernelGGL(gemm_grouped_f32<2>, dim3(tot), dim3(256), 0, st, djobs, dtile, dids,
+    hipLaunchKernelGGL((gemm_grouped<2, true>), dim3(tot), dim3(256), 0, st, djobs, dtile, dids,
                        (int)ids.size());
   else
-    hipLaunchKernelGGL(gemm_grouped_f32<4>, dim3(tot), dim3(256), 0, st, djobs, dtile, dids,
+    hipLaunchKernelGGL((gemm_grouped<4, true>), dim3(tot), dim3(256), 0, st, djobs, dtile, dids,
                        (int)ids.size());
   TN_CHECK_LAUNCH();
   return TN_OK;
@@ -575,6 +637,7 @@ static size_t batched_ws_bytes(const std::vector<int64_t>& ms, const std::vector
   a.take<double>(sv * 2);
   a.take<int>(sv);
   a.take<int64_t>((size_t)njobs);
+  a.take<int>(1);  // device step base
   return a.used + 4096;
 }
 
@@ -634,13 +697,10 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
     b.nchz = (int)ceil_div(k, BCH);
     b.gram_base = (int)g_job.size();
     b.eig_base = (int)e_job.size();
-    // The FP32 jobs' products are formed from FP32 factors with FP32
-    // accumulation (~1e-7 relative rounding): orthogonality beyond 1e-10 is
-    // below the data's precision, and orthogonalising its noise-level vectors
-    // to L * eps costs up to three times the sweeps.  FP64 matrices are
-    // orthogonalised to L * eps.
-    b.tol = tj ? std::max(1e-10, (double)L * 2.220446049250313e-16)
-               : (double)L * 2.220446049250313e-16;
+    // Orthogonality to L * eps (LAPACK dgesvj's threshold); the products of
+    // the truncation jobs are FP64-accurate (FP64-accumulated FP32 factors).
+    b.tol = (double)L * 2.220446049250313e-16;
+    b.neg2 = 0.0;
     for (int p = 0; p < b.nblk / 2; ++p) {
       for (int c = 0; c < b.nch; ++c) { g_job.push_back(j); g_pair.push_back(p); g_chunk.push_back(c); }
       e_job.push_back(j);
@@ -686,7 +746,8 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   double* dS = a.take<double>((size_t)sv);
   int* dperm = a.take<int>((size_t)sv);
   int64_t* dsbase = a.take<int64_t>((size_t)njobs);
-  TN_REQUIRE(dsbase != nullptr, "%s: workspace too small", who);
+  int* dgbase = a.take<int>(1);
+  TN_REQUIRE(dgbase != nullptr, "%s: workspace too small", who);
   for (int j = 0; j < njobs; ++j) gj[j].C = dWv + bj[j].wv;
   std::vector<int> gt(ngi * 3), et(nei * 2), at(nai * 3);
   for (int i = 0; i < ngi; ++i) { gt[i] = g_job[i]; gt[ngi + i] = g_pair[i]; gt[2 * ngi + i] = g_chunk[i]; }
@@ -703,6 +764,7 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   TN_HIP(hipMemsetAsync(ddone, 0, sizeof(int) * njobs, st));
   TN_HIP(hipMemsetAsync(dsweeps, 0, sizeof(int) * njobs, st));
   TN_HIP(hipMemsetAsync(doff, 0, sizeof(unsigned long long) * njobs, st));
+  TN_HIP(hipMemsetAsync(dgbase, 0, sizeof(int), st));
   // rows through the Gram and rotation kernels and eigensolves that rotated,
   // counted only while launch timing is on (they give the Jacobi rooflines)
   unsigned long long* cnt = prof_on() ? dwork : nullptr;
@@ -753,57 +815,124 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
       return b;
     };
     {
-      ProfScope p(PROF_GEMM_F32, st, work(use2), bytes(use2));
+      ProfScope p(PROF_GEMM_F64, st, work(use2), bytes(use2));
       TN_TRY(launch_grouped(2, use2, gj, dgj, dtile, dids, st));
     }
     if (!use4.empty()) {
       // second grouped launch needs its own tables: the first launch's are in use
       TN_HIP(hipStreamSynchronize(st));
-      ProfScope p(PROF_GEMM_F32, st, work(use4), bytes(use4));
+      ProfScope p(PROF_GEMM_F64, st, work(use4), bytes(use4));
       TN_TRY(launch_grouped(4, use4, gj, dgj, dtile, dids, st));
     }
   }
   // ---- 2. batched Jacobi
   hipLaunchKernelGGL(binit_identity, dim3((unsigned)ceil_div((int64_t)maxk * maxk, 256), njobs),
                      dim3(256), 0, st, dZ, dbj);
+  // truncation jobs: vectors negligible against the keep threshold
+  if (tj && rel_threshold > 0.0)
+    hipLaunchKernelGGL(bjob_neg2, dim3((unsigned)njobs), dim3(256), 0, st, dWv, dbj, rel_threshold);
   TN_CHECK_LAUNCH();
   // every job within one row chunk: the one-launch step
   bool fused = true;
   for (int j = 0; j < njobs; ++j) fused = fused && bj[j].L <= BCH && bj[j].k <= BCH;
-  const int poll = std::max(8, maxsteps);
-  int g = 0;
-  for (; g < maxsteps * max_sweeps; ++g) {
-    // work is counted on the device (cnt) and added once at the end
-    if (fused) {
-      ProfScope p(PROF_JAC_EIG, st, 0.0, 0.0);
-      hipLaunchKernelGGL(bjac_fused, dim3(nei), dim3(256), 0, st, dWv, dZ, dbj, det, det + nei,
-                         ddone, g, doff, cnt);
+  // A step set of `period` steps is enqueued as one unit (captured once as a
+  // HIP graph and replayed; launched kernel by kernel while launch timing is
+  // on); each set ends by advancing the device step base.  The host reads
+  // the active-job count of a set while the next set runs, so the GPU never
+  // waits for the host; a set enqueued after the last job retired exits at
+  // once in every block.
+  const int period = std::max(8, maxsteps);
+  auto enqueue_set = [&]() {
+    for (int o = 0; o < period; ++o) {
+      // work is counted on the device (cnt) and added once at the end
+      if (fused) {
+        ProfScope p(PROF_JAC_EIG, st, 0.0, 0.0);
+        hipLaunchKernelGGL(bjac_fused, dim3(nei), dim3(256), 0, st, dWv, dZ, dbj, det, det + nei,
+                           ddone, dgbase, o, doff, cnt);
+      } else {
+        {
+          ProfScope p(PROF_JAC_GRAM, st, 0.0, 0.0);
+          hipLaunchKernelGGL(bjac_gram, dim3(ngi), dim3(256), 0, st, dWv, dbj, dgt, dgt + ngi,
+                             dgt + 2 * ngi, ddone, dgbase, o, dG, cnt);
+        }
+        {
+          ProfScope p(PROF_JAC_EIG, st, 0.0, 0.0);
+          hipLaunchKernelGGL(bjac_eig, dim3(nei), dim3(256), 0, st, dG, dbj, det, det + nei, ddone,
+                             dQ, dskip, doff, cnt);
+        }
+        {
+          ProfScope p(PROF_JAC_APPLY, st, 0.0, 0.0);
+          hipLaunchKernelGGL(bjac_apply, dim3(nai), dim3(256), 0, st, dWv, dZ, dbj, dat, dat + nai,
+                             dat + 2 * nai, ddone, dgbase, o, dQ, dskip, cnt);
+        }
+      }
+      hipLaunchKernelGGL(bjac_sweep_end, dim3((unsigned)ceil_div(njobs, 256)), dim3(256), 0, st,
+                         dbj, njobs, dgbase, o, max_sweeps, ddone, dsweeps, doff, dactive);
+    }
+    hipLaunchKernelGGL(badvance, dim3(1), dim3(64), 0, st, dgbase, period);
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (!prof_on()) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      enqueue_set();
+      hipError_t e = hipStreamEndCapture(st, &graph);
+      if (e != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        graph = nullptr;
+        exec = nullptr;
+        (void)hipGetLastError();
+      }
     } else {
-    {
-      ProfScope p(PROF_JAC_GRAM, st, 0.0, 0.0);
-      hipLaunchKernelGGL(bjac_gram, dim3(ngi), dim3(256), 0, st, dWv, dbj, dgt, dgt + ngi,
-                         dgt + 2 * ngi, ddone, g, dG, cnt);
-    }
-    {
-      ProfScope p(PROF_JAC_EIG, st, 0.0, 0.0);
-      hipLaunchKernelGGL(bjac_eig, dim3(nei), dim3(256), 0, st, dG, dbj, det, det + nei, ddone, dQ,
-                         dskip, doff, cnt);
-    }
-    {
-      ProfScope p(PROF_JAC_APPLY, st, 0.0, 0.0);
-      hipLaunchKernelGGL(bjac_apply, dim3(nai), dim3(256), 0, st, dWv, dZ, dbj, dat, dat + nai,
-                         dat + 2 * nai, ddone, g, dQ, dskip, cnt);
-    }
-    }
-    hipLaunchKernelGGL(bjac_sweep_end, dim3((unsigned)ceil_div(njobs, 256)), dim3(256), 0, st, dbj,
-                       njobs, g, max_sweeps, ddone, dsweeps, doff, dactive);
-    TN_CHECK_LAUNCH();
-    if ((g + 1) % poll == 0) {
-      TN_HIP(hipMemcpyAsync(&h_active, dactive, sizeof(int), hipMemcpyDeviceToHost, st));
-      TN_HIP(hipStreamSynchronize(st));
-      if (h_active <= 0) break;
+      (void)hipGetLastError();
     }
   }
+  int* h_act = nullptr;  // pinned: the active count of the last two sets
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc_loop = TN_OK;
+  if (hipHostMalloc((void**)&h_act, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess) {
+    set_error(TN_ERR_HIP, "%s: host buffer or event creation failed", who);
+    rc_loop = TN_ERR_HIP;
+  }
+  const int nsets = (int)ceil_div((int64_t)maxsteps * max_sweeps, period);
+  for (int p = 0; rc_loop == TN_OK && p < nsets; ++p) {
+    if (exec) {
+      if (hipGraphLaunch(exec, st) != hipSuccess) {
+        set_error(TN_ERR_HIP, "%s: graph launch failed", who);
+        rc_loop = TN_ERR_HIP;
+        break;
+      }
+    } else {
+      enqueue_set();
+    }
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(h_act + (p & 1), dactive, sizeof(int), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipEventRecord(ev[p & 1], st) != hipSuccess) {
+      set_error(TN_ERR_HIP, "%s: step set %d failed to enqueue", who, p);
+      rc_loop = TN_ERR_HIP;
+      break;
+    }
+    if (p >= 1) {
+      if (hipEventSynchronize(ev[(p - 1) & 1]) != hipSuccess) {
+        set_error(TN_ERR_HIP, "%s: step set %d failed", who, p - 1);
+        rc_loop = TN_ERR_HIP;
+        break;
+      }
+      if (h_act[(p - 1) & 1] <= 0) break;
+    }
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (ev[0]) (void)hipEventDestroy(ev[0]);
+  if (ev[1]) (void)hipEventDestroy(ev[1]);
+  if (h_act) {
+    (void)hipStreamSynchronize(st);  // the last copy into h_act has landed
+    (void)hipHostFree(h_act);
+  }
+  if (rc_loop != TN_OK) return rc_loop;
   // ---- 3. sort, rank, outputs
   hipLaunchKernelGGL(bvec_norms, dim3((unsigned)maxk, njobs), dim3(256), 0, st, dWv, dbj, dsig,
                      dsbase);

@@ -148,6 +148,31 @@ __global__ void __launch_bounds__(256) sumsq_partial(const T* __restrict__ x, in
   if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// Counts of infinite and NaN entries (the non-finite diagnostic of a failed
+// SVD); one block, so no zero-initialised counters are needed.
+template <typename T>
+__global__ void __launch_bounds__(256) nonfinite_count(const T* __restrict__ x, int64_t n,
+                                                       unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[2][256];
+  unsigned long long ninf = 0, nnan = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    const double v = (double)x[i];
+    nnan += (v != v) ? 1ull : 0ull;
+    ninf += (v == v && (v - v) != (v - v)) ? 1ull : 0ull;
+  }
+  red[0][threadIdx.x] = ninf;
+  red[1][threadIdx.x] = nnan;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
 __global__ void __launch_bounds__(256) sum_final(const double* __restrict__ part, int n,
                                                  double* __restrict__ out) {
   __shared__ double red[4];
@@ -331,6 +356,25 @@ int tn_scale(void* x, int dtype, int64_t n, double alpha, void* stream) {
                        (float)alpha);
   else {
     set_error(TN_ERR_UNSUPPORTED, "tn_scale: unsupported dtype %d", dtype);
+    return TN_ERR_UNSUPPORTED;
+  }
+  TN_CHECK_LAUNCH();
+  return TN_OK;
+}
+
+int tn_count_nonfinite(const void* x, int dtype, int64_t n, unsigned long long* out,
+                       void* stream) {
+  TN_REQUIRE(n >= 0 && out != nullptr && (n == 0 || x != nullptr),
+             "tn_count_nonfinite: bad arguments");
+  hipStream_t st = as_stream(stream);
+  if (dtype == TN_F64)
+    hipLaunchKernelGGL(nonfinite_count<double>, dim3(1), dim3(256), 0, st, (const double*)x, n,
+                       out);
+  else if (dtype == TN_F32)
+    hipLaunchKernelGGL(nonfinite_count<float>, dim3(1), dim3(256), 0, st, (const float*)x, n,
+                       out);
+  else {
+    set_error(TN_ERR_UNSUPPORTED, "tn_count_nonfinite: unsupported dtype %d", dtype);
     return TN_ERR_UNSUPPORTED;
   }
   TN_CHECK_LAUNCH();

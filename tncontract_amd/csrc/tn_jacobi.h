@@ -50,14 +50,18 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
 }
 
 // max_{p<q} |G_pq| / sqrt(G_pp G_qq) over the 32 vectors (block-uniform).
-__device__ inline double eig_off(EigLds& sm) {
+// Pairs of two negligible vectors (squared norms <= neg2) are left out: the
+// caller has shown that the space they span lies below its truncation
+// threshold whether or not they are orthogonalised (neg2 = 0: every pair).
+__device__ inline double eig_off(EigLds& sm, double neg2 = 0.0) {
   const int tid = threadIdx.x;
   double off = 0.0;
   for (int e = tid; e < J2 * J2; e += 256) {
     int p = e / J2, q = e % J2;
     if (p < q) {
-      double d = sm.G[0][p][p] * sm.G[0][q][q];
-      if (d > 0.0) off = fmax(off, fabs(sm.G[0][p][q]) / sqrt(d));
+      const double gpp = sm.G[0][p][p], gqq = sm.G[0][q][q];
+      double d = gpp * gqq;
+      if (d > 0.0 && fmax(gpp, gqq) > neg2) off = fmax(off, fabs(sm.G[0][p][q]) / sqrt(d));
     }
   }
   off = wave_max(off);
@@ -90,15 +94,16 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // remove like any other (same sweep counts as an FP64 angle, measured).
 // Orthogonality comes from c = 1/sqrt(1 + t^2) in FP64 (v_rsq_f64 + two
 // Newton steps) and s = t c, so c^2 + s^2 = 1 to FP64 rounding.  A pair is
-// rotated when gpq^2 > tol^2 gpp gqq; the coefficients are computed without
+// rotated when gpq^2 > tol^2 gpp gqq and one of its vectors is not
+// negligible (squared norm > neg2; see eig_off); the coefficients are computed without
 // a branch (a pair that is not rotated selects c = 1, s = 0, discarding
 // whatever the formula gave), so the two rotations a thread needs per round
 // overlap.
 __device__ __forceinline__ void rot_coef(const double (*G)[J2 + 1], int x, int px, double tol,
-                                         double& a, double& b) {
+                                         double neg2, double& a, double& b) {
   const int l = min(x, px), h = max(x, px);
   const double gpq = G[l][h], gpp = G[l][l], gqq = G[h][h];
-  const bool rot = gpq * gpq > tol * tol * fmax(gpp * gqq, 0.0);
+  const bool rot = gpq * gpq > tol * tol * fmax(gpp * gqq, 0.0) && fmax(gpp, gqq) > neg2;
   const double D = gqq - gpp, P = 2.0 * gpq;
   const int e = __builtin_amdgcn_frexp_exp(fmax(fabs(D), fabs(P)));
   const float d = (float)__builtin_amdgcn_ldexp(fabs(D), -e);
@@ -121,7 +126,7 @@ __device__ __forceinline__ void rot_coef(const double (*G)[J2 + 1], int x, int p
 // needs the same number of sweeps with one inner sweep as with full inner
 // convergence (measured), at a fraction of the cost.  Returns the buffer
 // holding Q.
-__device__ inline int eig_rotate(EigLds& sm, double tol) {
+__device__ inline int eig_rotate(EigLds& sm, double tol, double neg2 = 0.0) {
   auto& G = sm.G;
   auto& Q = sm.Q;
   const int tid = threadIdx.x;
@@ -133,8 +138,8 @@ __device__ inline int eig_rotate(EigLds& sm, double tol) {
     const int nxt = cur ^ 1;
     // (c, b) for the first index of each pair; the partner's b is -b
     double cx, sx, cy, sy;
-    rot_coef(G[cur], x0, x1, tol, cx, sx);
-    rot_coef(G[cur], y0, y1, tol, cy, sy);
+    rot_coef(G[cur], x0, x1, tol, neg2, cx, sx);
+    rot_coef(G[cur], y0, y1, tol, neg2, cy, sy);
     const double bX0 = sx, bX1 = -sx, bY0 = sy, bY1 = -sy;
     const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
     const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];

@@ -10,6 +10,8 @@ reference itself reads.
 """
 from __future__ import annotations
 
+import warnings
+
 import numpy as np
 import torch
 
@@ -231,7 +233,14 @@ class MatrixProductState(OneDimensionalTensorNetwork):
             self.reverse()
         self.left_canonise(normalise=False, qr_decomposition=True)
         norm = self.norm(canonical_form="left")
-        self[-1].scale_(1.0 / norm)
+        if norm == 0.0:
+            # the reference divides by the zero norm (NaN data, numpy's
+            # "invalid value" warning); the right sweep's SVD then raises
+            # ValueError for the NaN matrix (tensor.py:922-929)
+            warnings.warn("invalid value encountered in divide", RuntimeWarning)
+            self[-1].scale_(float("nan"))
+        else:
+            self[-1].scale_(1.0 / norm)
         self.right_canonise(chi=chi, threshold=threshold, normalise=False)
         if not normalise:
             self[0].scale_(norm)

@@ -28,6 +28,19 @@ bond then depends on the left part through its Gram matrix alone.  So:
    Y_kept (right-isometric, labels [physout, right, left]);  P_i = C_i Y_kept.
 3. Site 0 = A_0 P_1 (labels [right, physout, left]).
 
+Resolution: K holds sigma^2, so a bond's eigenvalues are only resolved down
+to the eigensolver's Cholesky shift delta = 8 sqrt(n) eps_K tr K (eps_K the
+machine epsilon of the precision K was formed in); below sigma / sigma_0 =
+f = sqrt(4 delta / lambda_0) a value cannot be told from rounding (f ~ 1e-6
+for an FP64 K at n = 1024, ~1e-1 for an FP32 one).  A bond's rank is
+therefore taken here only when it is decided above f: by a threshold >= f,
+or by the chi / QR-bond cap binding above f.  A bond whose rank would be
+decided below f (a threshold-bound truncation of a decaying or rank-deficient
+spectrum, a chi cut deep in the tail, a zero state) sends the whole call to
+the exact path, contract_mps_mpo + svd_compress, so the result always has the
+reference's bond dimensions; ``SweepStats.fallback`` records the bond that
+did.
+
 Precision: "fp64" runs every contraction in FP64 (the algorithmic change
 alone); "fp32" / "bf16x3" run the contractions on FP32 / split-bf16 MFMA with
 FP32 accumulation, while K, the Jacobi rotations and the output isometries
@@ -56,6 +69,8 @@ class SweepStats:
         self.kept = {}      # bond i -> kept rank
         self.avail = {}     # bond i -> number of admissible singular values
         self.sweeps = {}    # bond i -> Jacobi sweeps of the eigensolver
+        self.floor = {}     # bond i -> resolution floor f of sigma / sigma_0
+        self.fallback = None  # bond whose rank sent the call to the exact path
 
 
 def _canon(t, labels):
@@ -89,8 +104,58 @@ def _cdt(x, dtype):
 MIXED_EIG_TOL = 1e-10
 
 
+# machine epsilon of K by the precision the contractions run in (bf16x3
+# products carry FP32-level rounding); the eigensolver's shift uses the same
+_EPS_K = {"fp64": 2.0 ** -52, "native": 2.0 ** -52, "fp32": 2.0 ** -23, "bf16x3": 2.0 ** -23}
+# eigenvalues below FLOOR_C * delta are taken as unresolved
+FLOOR_C = 4.0
+
+
+class _Unresolved(Exception):
+    """A bond's rank would be decided below the resolution floor."""
+
+    def __init__(self, bond):
+        super().__init__(bond)
+        self.bond = bond
+
+
+def bond_rank(lam, n, avail, chi, threshold, eps_k):
+    """Kept rank at a bond from the eigenvalues lam (descending, host) of its
+    Gram matrix, by the reference's rule (sigma / sigma_0 > threshold, then at
+    most chi; at most `avail`, the bond dimension the reference's QR sweep
+    gives), or None when that rank is decided below the resolution floor.
+    Returns (keep or None, rel sigma / sigma_0 over avail, floor f)."""
+    lam = np.asarray(lam, dtype=np.float64)
+    if lam.size == 0 or not np.isfinite(lam[0]) or lam[0] <= 0.0 or not np.all(np.isfinite(lam)):
+        return None, np.zeros(0), float("inf")
+    pos = np.maximum(lam, 0.0)
+    delta = 8.0 * np.sqrt(n) * eps_k * float(np.sum(pos))
+    f = float(np.sqrt(FLOOR_C * delta / lam[0]))
+    rel = np.sqrt(pos[:avail] / lam[0])
+    cap = min(chi, avail) if chi else avail
+    if threshold >= f:
+        return min(int(np.count_nonzero(rel > threshold)), cap), rel, f
+    if int(np.count_nonzero(rel > f)) >= cap:
+        return cap, rel, f
+    return None, rel, f
+
+
 def apply_mpo_compress(mps, mpo, chi=None, threshold=1e-15, precision="fp64", stats=None,
                        eig_tol=None):
+    """contract_mps_mpo(mps, mpo) followed by svd_compress(chi, threshold)
+    (onedim_core.py:1691, :463) by the structured sweep; falls back to exactly
+    those two calls when a bond's rank is not resolvable by the sweep."""
+    try:
+        return _apply_mpo_compress(mps, mpo, chi, threshold, precision, stats, eig_tol)
+    except _Unresolved as e:
+        if stats is not None:
+            stats.fallback = e.bond
+        out = core.contract_mps_mpo(mps, mpo)
+        out.svd_compress(chi=chi, threshold=threshold)
+        return out
+
+
+def _apply_mpo_compress(mps, mpo, chi, threshold, precision, stats, eig_tol):
     if len(mps) != len(mpo):
         raise ValueError("MPS and MPO have different lengths")
     N = len(mps)
@@ -138,21 +203,24 @@ def apply_mpo_compress(mps, mpo, chi=None, threshold=1e-15, precision="fp64", st
         # eigensolver's shift covers its rounding
         Km = Kt.permute(1, 0, 3, 2)
         info = {}
-        u, lam = tsr._eigh_psd_device(Km, 2, eig_tol, info)
+        try:
+            u, lam = tsr._eigh_psd_device(Km, 2, eig_tol, info)
+        except np.linalg.LinAlgError:
+            # no convergence (a zero or non-finite K among them): the exact
+            # path decides, with its own retry and diagnostics
+            raise _Unresolved(i)
         lam_h = lam.cpu().numpy()
         avail = min(n, qb[i])
-        sig = np.sqrt(np.maximum(lam_h[:avail], 0.0))
-        if sig.size == 0 or sig[0] == 0.0:
-            raise NotImplementedError("zero state in the structured sweep")
-        rel = sig / sig[0]
-        keep = int(np.count_nonzero(rel > threshold))
-        if chi:
-            keep = min(keep, chi)
+        keep, rel, f = bond_rank(lam_h, n, avail, chi, threshold, _EPS_K[precision])
         if stats is not None:
             stats.spectra[i] = rel
-            stats.kept[i] = keep
             stats.avail[i] = avail
             stats.sweeps[i] = info["sweeps"]
+            stats.floor[i] = f
+        if keep is None:
+            raise _Unresolved(i)
+        if stats is not None:
+            stats.kept[i] = keep
         if eig_tol > 0.0:
             # the rotations stopped at orthogonality eig_tol: one Newton-Schulz
             # step Y (3 I - Y^T Y) / 2 restores an isometry to rounding

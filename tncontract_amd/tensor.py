@@ -564,9 +564,31 @@ def _matrix_modes(d, nrow):
     return rows, cols, m, n
 
 
-def _svd_device(d, nrow):
+# Jacobi sweep limits of the SVD: the library's default for the first
+# attempt, and the retry that stands in for the reference's second LAPACK
+# driver (tensor.py:914-921) when the first attempt does not converge.
+SVD_MAX_SWEEPS = 40
+SVD_RETRY_SWEEPS = 160
+
+
+def _svd_nonfinite(d):
+    """tensor.py:922-929: report which non-finite entries the matrix holds and
+    raise ValueError (what scipy's gesvd raises for them)."""
+    ninf, nnan = L.count_nonfinite(d)
+    print("tensor_svd failed. Matrix contains inf's: " + str(ninf > 0)
+          + ". Matrix contains nan's: " + str(nnan > 0))
+    raise ValueError("array must not contain infs or NaNs")
+
+
+def _svd_device(d, nrow, max_sweeps=0):
     """Thin SVD of the (rows | cols) matrix view of d.  Returns (u, s, vt) with
-    u: m x k, s: k (float64, descending), vt: k x n, all on the device."""
+    u: m x k, s: k (float64, descending), vt: k x n, all on the device.
+
+    Failure handling follows tensor.py:914-929: a Jacobi that does not
+    converge in `max_sweeps` (0: SVD_MAX_SWEEPS) is retried once with
+    SVD_RETRY_SWEEPS after a warning, LinAlgError if that fails too; a matrix
+    with infinite or NaN entries prints the reference's diagnostic and raises
+    ValueError."""
     rows, cols, m, n = _matrix_modes(d, nrow)
     k = min(m, n)
     dt = d.dtype
@@ -578,10 +600,23 @@ def _svd_device(d, nrow):
     lib = L.lib()
     ws = L.workspace(lib.tn_svd_workspace(m, n, L.dtype_code(d)))
     sweeps = ctypes.c_int(0)
-    rc = lib.tn_svd(L.ptr(d), ctypes.byref(rows), ctypes.byref(cols), m, n, L.ptr(u), L.ptr(s),
-                    L.ptr(vt), L.dtype_code(d), 0.0, 0, ctypes.byref(sweeps), L.ptr(ws), ws.numel(),
-                    L.stream_ptr())
+
+    def run(limit):
+        return lib.tn_svd(L.ptr(d), ctypes.byref(rows), ctypes.byref(cols), m, n, L.ptr(u),
+                          L.ptr(s), L.ptr(vt), L.dtype_code(d), 0.0, int(limit),
+                          ctypes.byref(sweeps), L.ptr(ws), ws.numel(), L.stream_ptr())
+
+    limit = max_sweeps if max_sweeps > 0 else SVD_MAX_SWEEPS
+    rc = run(limit)
+    if rc == L.TN_ERR_NOCONV:
+        if any(L.count_nonfinite(d)):
+            _svd_nonfinite(d)
+        warnings.warn("tn_svd: the Jacobi SVD did not converge in %d sweeps, trying again with %d"
+                      % (limit, SVD_RETRY_SWEEPS))
+        rc = run(SVD_RETRY_SWEEPS)
     L.check(rc)
+    if not np.all(np.isfinite(s.cpu().numpy())):
+        _svd_nonfinite(d)
     return u, s, vt
 
 
