@@ -61,16 +61,22 @@ def parse():
                    help="Jacobi tolerance of the fast path's eigensolver (default: the "
                         "structured sweep's own, 1e-10 for mixed precision)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    p.add_argument("--no-extra", action="store_true",
+                   help="headline only: skip the config 3 and config 4 runs reported in its line")
     p.add_argument("--traffic", type=float, default=None,
                    help="HBM bytes per launch of the dominant kernel, from the calibrated "
                         "FETCH_SIZE + WRITE_SIZE counters (reported as roofline.traffic)")
-    p.add_argument("--workload", default="chain", choices=["chain", "batched", "variational", "peps", "tebd"],
+    p.add_argument("--workload", default="chain",
+                   choices=["chain", "batched", "variational", "peps", "tebd", "learned",
+                            "config5", "kernels"],
                    help="chain: the MPO.MPS compress sweep (default, the headline); batched: "
                         "1024 uneven contraction + truncation jobs (chi ~ U{16..512}, d=2, "
                         "FP32 data, keep sigma/sigma_0 > 1e-8); variational: iterations/s of "
                         "variational_compress on the MPO-applied chain; peps: boundary-MPS "
                         "inner_product_peps contractions/s; tebd: canonical-MPS two-site "
-                        "gates/s")
+                        "gates/s; learned: learned-rank compression sweeps/s (config 4); "
+                        "config5: chi=1024 bf16x3 sweeps, 4 in a row; kernels: the hot "
+                        "kernels at the headline shapes, for counter profiles")
     p.add_argument("--tebd-N", type=int, default=64)
     p.add_argument("--tebd-chi", type=int, default=128)
     p.add_argument("--peps-L", type=int, default=10)
@@ -115,10 +121,15 @@ def flops_per_site_plain(chi_l, chi_r, D_l, D_r, d, chi_out_r, qb_l):
     return qr + gemm + svd
 
 
-def cpu_baseline(N, d, chi, D=5):
-    """Time the oracle on one bulk site of the workload; scale to a sweep."""
+def cpu_baseline(N, d, chi, D=5, window=4):
+    """Time the oracle on `window` consecutive bulk sites of the workload's
+    sweep (the reference algorithm's steps in order: MPO application of each
+    site, then the QR left sweep over the window carrying R from site to site,
+    then the truncating SVD right sweep back over it carrying S V) and scale
+    to the sweep by the closed-form per-site flop model."""
     import numpy as np
     from oracle import tn_oracle as o
+    from oracle import models as om
     try:
         from threadpoolctl import threadpool_info
         blas = [(x.get("internal_api"), x.get("num_threads")) for x in threadpool_info()]
@@ -127,30 +138,43 @@ def cpu_baseline(N, d, chi, D=5):
     cores = len(os.sched_getaffinity(0))
     rng = np.random.default_rng(1)
     chiD = chi * D
-    # one bulk site of the plain sweep, as the oracle executes it
-    B = o.OTensor(rng.standard_normal((d, chi, chi)), ["phys", "left", "right"])
-    W = o.OTensor(rng.standard_normal((D, D, d, d)), ["left", "right", "physout", "physin"])
-    site = o.OTensor(rng.standard_normal((chiD, d, chiD)), ["left", "physout", "right"])
-    Rprev = o.OTensor(rng.standard_normal((chiD, chiD)), ["qr_out", "right"])
+    W = o.OTensor(om.xxz_bulk_w(), ["left", "right", "physout", "physin"])
+    Bs = [o.OTensor(rng.standard_normal((d, chi, chi)) / np.sqrt(2 * chi),
+                    ["phys", "left", "right"]) for _ in range(window)]
+    # R of the QR sweep entering the window (upper triangular, chi D x chi D)
+    R = o.OTensor(np.triu(rng.standard_normal((chiD, chiD))), ["qr_out", "left"])
     t0 = time.perf_counter()
-    A = o.contract(B, W, "phys", "physin")
-    A.consolidate_indices()                                           # MPO application
+    sites = []
+    for B in Bs:                                                       # MPO application
+        A = o.contract(B, W, "phys", "physin")
+        A.consolidate_indices()                                        # (left, physout, right)
+        sites.append(A)
     t_mpo = time.perf_counter() - t0
     t0 = time.perf_counter()
-    nxt = o.contract(Rprev, site, "right", "left")                    # R . next site
-    nxt.replace_label("qr_out", "left")
-    Q, R = o.tensor_qr(nxt, ["physout", "left"])                      # QR left sweep
+    for i in range(window):                                            # QR left sweep
+        nxt = o.contract(R, sites[i], "left", "left")
+        nxt.replace_label("qr_out", "left")
+        Q, R = o.tensor_qr(nxt, ["physout", "left"], qr_label="qr_")
+        Q.replace_label("qr_in", "right")
+        sites[i] = Q
+        R.replace_label("right", "left")
     t_qr = time.perf_counter() - t0
-    c = o.OTensor(rng.standard_normal((d, chi, chiD)), ["physout", "left", "right"])
+    # right sweep: the window's last site already carries the truncated
+    # chi-bond part of the sites to its right
+    SV = o.OTensor(np.linalg.qr(rng.standard_normal((chiD, chi)))[0], ["left", "right"])
     t0 = time.perf_counter()
-    U, S, V = o.tensor_svd(c, ["physout", "left"])                    # right sweep SVD
-    V.data = V.data[:chi]
-    S.data = S.data[:chi, :chi]
-    Q.replace_label([l for l in Q.labels if l.endswith("in")][0], "right")
-    nv = o.contract(V, Q, "right", "right")                            # S V . site
-    nv = o.contract(S, nv, "svd_in", "svd_out")
+    for i in range(window - 1, -1, -1):                                # truncating SVD sweep
+        c = o.contract(sites[i], SV, "right", "left")                  # (left, physout, right=chi)
+        U, S, V = o.tensor_svd(c, ["physout", "right"])
+        U.data = U.data[..., :chi]
+        S.data = S.data[:chi, :chi]
+        V.data = V.data[:chi]
+        SV = o.contract(S, V, "svd_in", "svd_out")                     # (svd_out, left)
+        SV.replace_label(["svd_out", "left"], ["right", "left"])
+        SV.data = SV.data.T.copy()
+        SV.labels = ["left", "right"]
     t_svd = time.perf_counter() - t0
-    t_bulk = t_mpo + t_qr + t_svd
+    t_win = t_mpo + t_qr + t_svd
     # scale by the per-site flop model over the sweep's actual shapes
     qb = [1]  # left bond after the QR sweep: min(d * previous, chi * D)
     for i in range(N - 1):
@@ -159,13 +183,38 @@ def cpu_baseline(N, d, chi, D=5):
     total = 0.0
     for i in range(1, N - 1):
         total += flops_per_site_plain(chi, chi, D, D, d, chi, qb[i]) / bulk
-    sweep_s = t_bulk * max(total, 1.0)
+    sweep_s = t_win / window * max(total, 1.0)
     return {"value": 1.0 / sweep_s, "unit": "sweeps/s", "cores": cores, "kind": "port",
-            "sample": ("NumPy FP64 oracle, one bulk site of the reference algorithm at chi=%d, "
-                       "D=%d (MPO application %.2fs, R.site + QR %.2fs, SVD + absorb %.2fs), "
-                       "scaled to the %d-site sweep by the closed-form per-site flop model "
-                       "(%.1f bulk-site equivalents); BLAS %s"
-                       % (chi, D, t_mpo, t_qr, t_svd, N, total, blas))}
+            "sample": ("NumPy FP64 oracle, %d consecutive bulk sites of the reference sweep at "
+                       "chi=%d, D=%d (MPO application %.2fs, QR left sweep with R carried %.2fs, "
+                       "truncating SVD right sweep %.2fs; %.2fs per site), scaled to the %d-site "
+                       "sweep by the closed-form per-site flop model (%.1f bulk-site "
+                       "equivalents); BLAS %s"
+                       % (window, chi, D, t_mpo, t_qr, t_svd, t_win / window, N, total, blas))}
+
+
+# FP64-equivalent peaks (TFLOP/s) of the tagged kernels for the sweep-level
+# roofline; bf16x3 takes three BF16 MFMAs per FP32-equivalent product
+SWEEP_PEAKS = {"gemm_f64": 78.6, "gemm_f32": 157.3, "gemm_bf16x3": 2500.0 / 3.0,
+               "jac_gram": 78.6, "jac_eig_apply": 78.6, "jac_apply": 78.6, "qr_panel": 78.6,
+               "direct": None}
+
+
+def sweep_roofline(prof, t_measured_s):
+    """T_min = sum over the tagged kernels of max(F_k / P_k, B_k / 8 TB/s)
+    (F_k, B_k: the algorithmic flops and bytes the library counted for them
+    in one profiled sweep) against the measured sweep time."""
+    t_min = 0.0
+    for tag, (ms, n, fl, by) in prof.items():
+        if not n:
+            continue
+        peak = SWEEP_PEAKS.get(tag)
+        tf = fl / (peak * 1e12) if peak else 0.0
+        t_min += max(tf, by / 8e12)
+    return {"t_min_ms": t_min * 1e3, "t_measured_ms": t_measured_s * 1e3,
+            "frac": t_min / t_measured_s if t_measured_s > 0 else None,
+            "note": "T_min over the kernels the library times (GEMMs, Jacobi, QR panels, "
+                    "thin-K contractions) at their closed-form work"}
 
 
 def batched_workload(args):
@@ -189,12 +238,14 @@ def batched_workload(args):
         U, S, Vt, kept, sweeps = batched_truncate(Ls, Rs, rel_threshold=1e-8)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
-    # parity sample: singular values of 8 jobs against LAPACK on the same data
-    err = 0.0
+    # parity sample: kept rank and singular values of 8 jobs against LAPACK
+    # on the same data (the oracle's FP64 product of the FP32 factors)
+    err, kept_ok = 0.0, True
     for j in range(0, len(jobs), max(1, len(jobs) // 8)):
         C = jobs[j][0].astype(np.float64) @ jobs[j][1].astype(np.float64)
         ref = np.linalg.svd(C, compute_uv=False)
         k = int(kept[j])
+        kept_ok = kept_ok and k == int(np.count_nonzero(ref / ref[0] > 1e-8))
         err = max(err, float(np.max(np.abs(S[j].cpu().numpy()[:k] - ref[:k])) / ref[0]))
     cpu = None
     if not args.no_cpu:
@@ -223,15 +274,16 @@ def batched_workload(args):
     rec = {"metric": "batched uneven contraction + truncation jobs/s", "value": len(jobs) / dt,
            "unit": "jobs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32+f64",
+           "vs_baseline": None, "dtype": "f64 (FP32 factors)",
            "data": "synthetic (seeded factors with exp(-k/tau) spectra)",
            "config": {"workload": "%d jobs, chi ~ U{16..512}, d=2, keep sigma/sigma_0 > 1e-8"
                                   % len(jobs)},
            "jacobi_sweeps_max": int(np.max(sweeps)), "kept_mean": float(np.mean(kept)),
+           "kept_equal_lapack_sample": kept_ok,
            "gemm_gflop": flops / 1e9, "sv_rel_err_sample": err,
            "kernel_ms_profiled_step": {t: round(v[0], 3) for t, v in prof.items() if v[1]},
            "roofline": roof, "cpu_baseline": cpu}
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 def roofline_from_prof(prof, traffic=None, measured=None):
@@ -361,7 +413,7 @@ def variational_workload(args):
            "projection_identity_rel_err": proj_err,
            "kernel_ms_profiled_iteration": {t: round(v[0], 3) for t, v in prof.items() if v[1]},
            "roofline": roof, "cpu_baseline": cpu}
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 def peps_arrays(L, D, d, seed):
@@ -443,7 +495,240 @@ def peps_workload(args):
            "value_norm2": value, "rel_err_vs_oracle": rel_err,
            "kernel_ms_profiled_step": {t: round(v[0], 3) for t, v in prof.items() if v[1]},
            "roofline": roof, "cpu_baseline": cpu}
-    print(json.dumps(rec), flush=True)
+    return rec
+
+
+def kernels_workload(args):
+    """The headline sweep's hot kernels at their shapes, each launched a few
+    times on resident data (the driver for the PMC counter profiles):
+      gemm_f64 / gemm_f32 / gemm_bf16x3: the R . next-site contraction of the
+        plain sweep, (2560 x 2560) . (2560 x 2 x 2560), FP64 / FP32 / bf16x3;
+      permute: a (2560, 2, 2560) FP64 site moved to (2, 2560, 2560) order;
+      direct: the MPO application of one site, (2, 512, 512) x XXZ W ->
+        (2560, 2, 2560) with the consolidate_indices layout (contract_mps_mpo).
+    Prints one JSON line with each kernel's average time, algorithmic work and
+    rate from the library's launch timers."""
+    import numpy as np
+    import torch
+    import tncontract_amd as tn
+    from tncontract_amd import _lib as L
+    from tncontract_amd import tensor as tsr
+    from tncontract_amd.onedim import xxz_mpo
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(args.seed)
+    chi, D, d = args.chi, 5, 2
+    cd = chi * D
+    R64 = torch.from_numpy(rng.standard_normal((cd, cd))).cuda()
+    S64 = torch.from_numpy(rng.standard_normal((cd, d, cd))).cuda()
+    B = tn.Tensor(rng.standard_normal((d, chi, chi)), ["phys", "left", "right"])
+    W = xxz_mpo(3)[1]
+    reps = max(args.steps, 3)
+    out = {}
+
+    def timed(tag, fn, flops, nbytes):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[tag] = {"ms": dt * 1e3, "flops": flops, "bytes": nbytes,
+                    "tflops": flops / dt / 1e12 if flops else None,
+                    "gbs": nbytes / dt / 1e9}
+        log("%s: %.3f ms" % (tag, dt * 1e3))
+
+    fl = 2.0 * cd * cd * d * cd
+    for prec in ("native", "fp32", "bf16x3"):
+        tag = {"native": "gemm_f64", "fp32": "gemm_f32", "bf16x3": "gemm_bf16x3"}[prec]
+        timed(tag, lambda: tsr._contract_data(R64, [1], S64, [0], prec), fl,
+              8.0 * (cd * cd + 2 * cd * d * cd))
+    Sv = S64.permute(1, 0, 2)
+    timed("permute", lambda: L.materialize(Sv), 0.0, 16.0 * S64.numel())
+    timed("direct", lambda: tn.contract(B, W, "phys", "physin"), 2.0 * d * cd * d * cd,
+          8.0 * (B.data.numel() + cd * d * cd))
+    rec = {"metric": "hot-kernel rates at the chi=%d sweep's shapes" % chi, "value": None,
+           "unit": None, "n_gpus": 1, "steps": reps, "warmup": 1, "higher_is_better": True,
+           "data": "synthetic (seeded N(0,1))",
+           "config": {"workload": "contraction GEMMs (%d x %d) . (%d x %d x %d), permute of a "
+                                  "(%d, %d, %d) site, MPO application of a (%d, %d, %d) site"
+                                  % (cd, cd, cd, d, cd, cd, d, cd, d, chi, chi)},
+           "kernels": out}
+    return rec
+
+
+def config5_workload(args):
+    """Config 5: XXZ MPO (D=5) applied to a seeded random MPS N=128, d=2,
+    chi=1024 (fused bond 5120) and compressed back to chi=1024, four times in
+    a row (each sweep's output is the next one's input), bf16x3 contractions
+    with FP64 eigensolver rotations and isometries.  One step = one sweep.
+    Reports the peak device memory and, for the first sweep, the state
+    distance to the FP32-contraction sweep.  The oracle is not run (about
+    209 TFLOP per sweep on the host)."""
+    import torch
+    import tncontract_amd as tn
+    from tncontract_amd import _lib as L
+    from tncontract_amd.onedim import random_mps_arrays, xxz_mpo
+    from tncontract_amd.onedim.structured import apply_mpo_compress, SweepStats
+    torch.cuda.set_device(0)
+    N, d, chi = 128, 2, 1024
+    arrs = random_mps_arrays(N, d, chi, seed=args.seed)
+    psi = tn.onedim.MatrixProductState([tn.Tensor(a, ["phys", "left", "right"]) for a in arrs])
+    mpo = xxz_mpo(N)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    log("config5: first sweep (bf16x3), then the FP32 one for the distance")
+    t0 = time.perf_counter()
+    first = apply_mpo_compress(psi, mpo, chi=chi, threshold=1e-15, precision="bf16x3")
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    ref = apply_mpo_compress(psi, mpo, chi=chi, threshold=1e-15, precision="fp32")
+    ip = tn.onedim.inner_product_mps
+    nf = float(ip(ref, ref))
+    dist = (float(ip(first, first)) + nf - 2 * float(ip(first, ref))) / nf
+    del ref
+    steps = max(args.steps, 1)
+    log("config5: %d further sweeps" % steps)
+    cur = first
+    st = SweepStats()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        if k == steps - 1:
+            L.prof_collect("gemm_f64", reset=True)
+            L.prof_enable(True)
+        cur = apply_mpo_compress(cur, mpo, chi=chi, threshold=1e-15, precision="bf16x3",
+                                 stats=st if k == steps - 1 else None)
+        torch.cuda.synchronize()
+        log("config5: sweep %d done, %.1f s" % (k + 2, time.perf_counter() - t0))
+    L.prof_enable(False)
+    prof = {tag: L.prof_collect(tag) for tag in L.PROF_TAGS}
+    L.prof_collect("gemm_f64", reset=True)
+    dt = (time.perf_counter() - t0) / steps
+    peak = torch.cuda.max_memory_allocated()
+    roof = roofline_from_prof(prof, args.traffic)
+    rec = {"metric": "MPO.MPS compress sweeps/s at chi=1024 (config 5)", "value": 1.0 / dt,
+           "unit": "sweeps/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16x3+f64",
+           "data": "synthetic (seeded random MPS, i.i.d. N(0,1), normalised; XXZ MPO)",
+           "config": {"workload": "XXZ MPO D=5 x random MPS N=128 d=2 chi=1024 -> compress "
+                                  "chi=1024, %d sweeps after the first, each on the previous "
+                                  "output; bf16x3 contractions" % steps},
+           "first_sweep_s": t_first, "peak_device_memory_gb": peak / 1e9,
+           "first_sweep_state_dist_bf16x3_vs_fp32": dist,
+           "eig_jacobi_sweeps_mean": sum(st.sweeps.values()) / max(len(st.sweeps), 1),
+           "structured_fallback_bond": st.fallback,
+           "kernel_ms_profiled_step": {t: round(v[0], 3) for t, v in prof.items() if v[1]},
+           "roofline": roof,
+           "cpu_baseline": None,
+           "cpu_baseline_note": "not run: about 209 TFLOP of FP64 per sweep on the host"}
+    return rec
+
+
+def learned_workload(args):
+    """Config 4: the truncating sweep of svd_compress on a d=4 chain whose
+    input bond is 1024 (bond matrices up to 2048 x 1024), compressed to
+    chi=512 at sigma/sigma_0 > 1e-8, FP32 data in FP64 arithmetic.  The rank
+    model is trained on the GPU on the spectra of full-SVD sweeps of other
+    states of the family and of config 1; one step = one learned-rank sweep;
+    the same sweep with the full Jacobi SVD at every bond (the method without
+    the network) is timed beside it."""
+    import numpy as np
+    import torch
+    from tncontract_amd import _lib as L
+    from tncontract_amd import learned as lr
+    torch.cuda.set_device(0)
+    chi, thr = 512, 1e-8
+    log("learned: training sweeps")
+    t0 = time.perf_counter()
+    samples = []
+    for seed in (1, 2, 3, 4):
+        prev, cur, _ = lr.decaying_pair(seed)
+        samples.append(lr.sweep_samples(prev, cur, chi, thr)[:2])
+    for seed in (1, 2):
+        prev, cur = lr.flat_pair(seed, 20, 2, 32)
+        samples.append(lr.sweep_samples(prev, cur, 16, 1e-10)[:2])
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    model, losses = lr.train_rank_model(samples)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    prev, cur, taus = lr.decaying_pair(99 + args.seed)
+    X, T, spectra_prev, needed_cur = lr.sweep_samples(prev, cur, chi, thr)
+    pred = model.predict_rank(X, chi)
+    rank_err = float(np.mean(np.abs(pred - T * chi)) / chi)
+    state = lr._state(cur)
+    torch.cuda.synchronize()
+
+    def learned_step(stats=None):
+        return lr.learned_svd_compress(state, chi, thr, model, spectra_prev, stats=stats)
+
+    def full_step():
+        return lr.record_sweep(state, chi, thr)[0]
+
+    log("learned warmup")
+    for _ in range(args.warmup):
+        learned_step()
+        full_step()
+    torch.cuda.synchronize()
+    log("learned timed x%d" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = learned_step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ref = full_step()
+    torch.cuda.synchronize()
+    dt_full = (time.perf_counter() - t0) / args.steps
+    st = lr.LearnedSweepStats()
+    L.prof_collect("gemm_f64", reset=True)
+    L.prof_enable(True)
+    learned_step(st)
+    torch.cuda.synchronize()
+    L.prof_enable(False)
+    prof = {tag: L.prof_collect(tag) for tag in L.PROF_TAGS}
+    L.prof_collect("gemm_f64", reset=True)
+    roof = roofline_from_prof(prof, args.traffic)
+    ip = __import__("tncontract_amd").onedim.inner_product_mps
+    nn = float(ip(ref, ref))
+    err = float((float(ip(out, out)) + nn - 2 * float(ip(out, ref))) / nn)
+    cpu, err_oracle = None, None
+    if not args.no_cpu:
+        from oracle import tn_oracle as o
+        log("cpu baseline: oracle svd_compress of the same state")
+        r = o.OMPS([o.OTensor(a.astype(np.float64), ["phys", "left", "right"]) for a in cur])
+        t1 = time.perf_counter()
+        r.svd_compress(chi=chi, threshold=thr)
+        tc = time.perf_counter() - t1
+        g = o.OMPS([o.OTensor(t.numpy(), t.labels) for t in out], out.left_label,
+                   out.right_label, out.phys_label)
+        nr = o.inner_product_mps(r, r)
+        err_oracle = float((o.inner_product_mps(g, g) + nr - 2 * o.inner_product_mps(g, r)) / nr)
+        cpu = {"value": 1.0 / tc, "unit": "sweeps/s", "cores": len(os.sched_getaffinity(0)),
+               "kind": "port", "sample": "NumPy FP64 oracle svd_compress of the same state once "
+                                         "(%.2f s)" % tc}
+    rec = {"metric": "learned-rank compression sweeps/s (config 4)", "value": 1.0 / dt,
+           "unit": "sweeps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 (FP32 data)",
+           "data": "synthetic (seeded d=4 MPS, input bond 1024, bond spectra ~exp(-k/tau), "
+                   "tau ~ U[4,64])",
+           "config": {"workload": "svd_compress to chi=512 at sigma/sigma_0 > 1e-8, N=16, d=4, "
+                                  "input bond 1024 (bond matrices up to 2048 x 1024); rank model "
+                                  "trained on %d sweep samples" % sum(len(x[1]) for x in samples)},
+           "full_svd_sweeps_per_s": 1.0 / dt_full, "speedup_vs_full_svd_sweep": dt_full / dt,
+           "state_err_vs_full_svd_sweep": err, "state_err_vs_oracle": err_oracle,
+           "bond_dims_equal_full_svd": out.bonddims() == ref.bonddims(),
+           "fallbacks": st.fallbacks, "paths": {str(b): p for b, p in sorted(st.path.items())},
+           "predictor_rank_error_mean_over_chi": rank_err,
+           "columns_factored": st.work_cols, "columns_full_svd": st.full_cols,
+           "train": {"samples": int(sum(len(x[1]) for x in samples)), "data_s": t_data,
+                     "train_s": t_train, "loss_first": losses[0], "loss_last": losses[-1]},
+           "kernel_ms_profiled_step": {t: round(v[0], 3) for t, v in prof.items() if v[1]},
+           "roofline": roof, "cpu_baseline": cpu}
+    return rec
 
 
 def xxz_bond_gate(tau, Jz=1.0, Jxy=1.0):
@@ -554,7 +839,7 @@ def tebd_workload(args):
            "mid_bond_lambda_rel_err_vs_oracle": lam_err,
            "kernel_ms_profiled_step": {t: round(v[0], 3) for t, v in prof.items() if v[1]},
            "roofline": roof, "cpu_baseline": cpu}
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 def main():
@@ -566,18 +851,19 @@ def main():
         print("bench.py: measures one GPU (the sweep does not shard within a chain); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    if args.workload == "batched":
-        batched_workload(args)
+    others = {"batched": batched_workload, "variational": variational_workload,
+              "peps": peps_workload, "tebd": tebd_workload, "learned": learned_workload,
+              "config5": config5_workload, "kernels": kernels_workload}
+    if args.workload in others:
+        print(json.dumps(others[args.workload](args)), flush=True)
         return
-    if args.workload == "variational":
-        variational_workload(args)
-        return
-    if args.workload == "peps":
-        peps_workload(args)
-        return
-    if args.workload == "tebd":
-        tebd_workload(args)
-        return
+    print(json.dumps(chain_workload(args)), flush=True)
+
+
+def chain_workload(args):
+    """The headline: MPO.MPS compress sweeps/s at chi=512 (see the module
+    docstring); the batched (config 3) and learned-rank (config 4) workloads
+    run after it with their own steps and are reported inside its record."""
     import numpy as np
     import torch
 
@@ -627,6 +913,7 @@ def main():
     L.prof_collect("gemm_f64", reset=True)
     roof = roofline_from_prof(prof, args.traffic, MEASURED_TRAFFIC)
     kernel_ms = {t: round(v[0], 3) for t, v in prof.items() if v[1]}
+    sweep_roof = sweep_roofline(prof, dt_fast)
 
     # ---- plain FP64 path (the denominator) and the error between the two
     dt_plain, err = None, None
@@ -667,17 +954,40 @@ def main():
         "data": "synthetic (seeded random MPS, i.i.d. N(0,1), normalised; XXZ MPO)",
         "config": {"workload": "XXZ MPO D=5 x random MPS N=%d d=%d chi=%d -> compress chi=%d"
                                % (N, d, chi, chi),
-                   "fast_path": "structured apply+compress, %s contractions, FP64 Jacobi"
-                                % args.precision},
+                   "fast_path": ("structured apply+compress (no chi*D chain, no QR sweep), %s "
+                                 "contractions, FP64 Cholesky + one-sided Jacobi eigensolver of "
+                                 "each bond's Gram matrix, FP64 isometries; the learned-rank "
+                                 "randomized SVD is not in this timed path (every bond of this "
+                                 "flat-spectrum workload keeps half of its Gram's rank, where a "
+                                 "sketch is no smaller than the full solve) and is measured by "
+                                 "--workload learned" % args.precision)},
         "plain_fp64_sweeps_per_s": (1.0 / dt_plain) if dt_plain else None,
         "speedup_vs_plain_fp64": (dt_plain / dt_fast) if dt_plain else None,
         "state_err_vs_plain_fp64": err,
         "kernel_ms_profiled_sweep": kernel_ms,
         "eig_jacobi_sweeps_mean": (sum(st.sweeps.values()) / max(len(st.sweeps), 1)),
+        "structured_fallback_bond": st.fallback,
+        "sweep_roofline": sweep_roof,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(rec), flush=True)
+    if not args.no_extra:
+        ex = argparse.Namespace(**vars(args))
+        ex.steps, ex.warmup, ex.traffic, ex.seed = 2, 1, None, 0
+        keys = ("value", "unit", "ms_per_step", "config", "roofline", "cpu_baseline")
+        log("config 3: batched truncation")
+        b = batched_workload(ex)
+        rec["batched_config3"] = dict({k: b[k] for k in keys}, kept_vs_lapack_sample=b.get(
+            "kept_equal_lapack_sample"), sv_rel_err_sample=b.get("sv_rel_err_sample"),
+            jacobi_sweeps_max=b.get("jacobi_sweeps_max"))
+        log("config 4: learned-rank sweep")
+        lw = learned_workload(ex)
+        rec["learned_config4"] = dict({k: lw[k] for k in keys}, **{
+            k: lw[k] for k in ("full_svd_sweeps_per_s", "speedup_vs_full_svd_sweep",
+                               "state_err_vs_oracle", "fallbacks", "paths",
+                               "predictor_rank_error_mean_over_chi", "columns_factored",
+                               "columns_full_svd")})
+    return rec
 
 
 if __name__ == "__main__":

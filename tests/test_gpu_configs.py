@@ -82,7 +82,9 @@ def config2(tn):
     arrs = random_mps_arrays(N, 2, chi, seed=2)
     r = o.contract_mps_mpo(oracle_state(arrs), om.xxz_mpo(N))
     r.svd_compress(chi=chi, threshold=1e-15)
-    sz = o.expvals_mps(r, o.OTensor(om.SZ, ["out", "in"])).real
+    # expvals_mps moves the orthogonality centre in place (onedim_utils.py:
+    # 223-228): a copy keeps r as svd_compress left it
+    sz = o.expvals_mps(r.copy(), o.OTensor(om.SZ, ["out", "in"])).real
     return arrs, r, sz
 
 
@@ -114,8 +116,8 @@ def test_config2_mixed_precision(tn, config2, prec):
     assert [t.labels for t in g] == [t.labels for t in r]
     err = rel_dist2(to_oracle(g), r)
     assert err < 1e-8, err
-    assert np.max(np.abs(_sz_gpu(tn, g) - sz)) < 1e-4
     assert g.check_canonical_form(threshold=1e-10, print_output=False)[1] == 0
+    assert np.max(np.abs(_sz_gpu(tn, g) - sz)) < 1e-4
 
 
 def test_headline_shapes_chi512(tn):
@@ -142,3 +144,32 @@ def test_headline_shapes_chi512(tn):
     assert f.bonddims() == r.bonddims()
     err = rel_dist2(to_oracle(f), r)
     assert err < 1e-8, err
+
+
+def test_config5_shapes_bf16x3(tn):
+    """Config 5's per-site shapes (XXZ D=5 on chi=1024, fused bond 5120,
+    compressed back to 1024) on a 24-site chain with bf16x3 contractions:
+    the structured sweep's result is right-canonical to 1e-10 and within
+    1e-8 (state distance) of the FP32-contraction sweep; its peak device
+    memory is printed.  (The 128-site chain itself
+    is the benchmark's workload; the oracle is not run at this size.)"""
+    from tncontract_amd.onedim import random_mps_arrays, xxz_mpo
+    from tncontract_amd.onedim.structured import apply_mpo_compress, SweepStats
+    N, chi = 24, 1024
+    arrs = random_mps_arrays(N, 2, chi, seed=5)
+    psi = gpu_state(tn, arrs)
+    torch.cuda.reset_peak_memory_stats()
+    st = SweepStats()
+    b = apply_mpo_compress(psi, xxz_mpo(N), chi=chi, threshold=1e-15, precision="bf16x3", stats=st)
+    peak = torch.cuda.max_memory_allocated()
+    assert st.fallback is None
+    assert max(st.avail.values()) == 2 * chi          # the n = 2048 eigensolve ran
+    assert max(b.bonddims()) == chi
+    assert b.check_canonical_form(threshold=1e-10, print_output=False)[1] == 0
+    f = apply_mpo_compress(psi, xxz_mpo(N), chi=chi, threshold=1e-15, precision="fp32")
+    assert f.bonddims() == b.bonddims()
+    ip = tn.onedim.inner_product_mps
+    nf = float(ip(f, f))
+    dist = (float(ip(b, b)) + nf - 2 * float(ip(b, f))) / nf
+    assert abs(dist) < 1e-8, dist
+    print("config 5 shapes: peak device memory %.2f GB" % (peak / 1e9))

@@ -17,26 +17,60 @@ def test_bond_rank_follows_the_reference_rule_above_the_floor():
     np.testing.assert_allclose(rel, np.exp(-np.arange(64) / 4.0), rtol=1e-14)
     assert f < 2e-6
     assert bond_rank(lam, 64, 64, 12, 1e-15, _EPS_K["fp64"])[0] == 12
-    assert bond_rank(lam, 64, 10, 12, 1e-15, _EPS_K["fp64"])[0] == 10   # avail caps
+    # avail caps: the rank bound 10, K's eigenvalues past it rounding-level
+    capped = np.concatenate([lam[:10], 1e-18 * np.ones(54)])
+    assert bond_rank(capped, 64, 10, 12, 1e-15, _EPS_K["fp64"])[0] == 10
 
 
 def test_bond_rank_refuses_decisions_below_the_floor():
     lam = np.exp(-np.arange(64) / 2.0)
-    # FP32-formed K: floor ~1e-2, so a cut at 1e-3 or a threshold of 2e-6 is unresolved
-    assert bond_rank(lam, 64, 64, 28, 1e-15, _EPS_K["fp32"])[0] is None
-    assert bond_rank(lam, 64, 64, None, 2e-6, _EPS_K["fp32"])[0] is None
-    assert bond_rank(lam, 64, 64, 12, 1e-15, _EPS_K["fp32"])[0] == 12
-    # rank-deficient Gram (exact zeros plus rounding noise): threshold-bound
+    # an FP32-formed K whose asymmetry puts the floor near 2e-2: a cut at
+    # 1e-3 or a threshold of 2e-6 is unresolved, a cut at 6e-2 is not
+    assert bond_rank(lam, 64, 64, 28, 1e-15, _EPS_K["fp32"], asym=1e-5)[0] is None
+    assert bond_rank(lam, 64, 64, None, 2e-6, _EPS_K["fp32"], asym=1e-5)[0] is None
+    assert bond_rank(lam, 64, 64, 12, 1e-15, _EPS_K["fp32"], asym=1e-5)[0] == 12
+    # rank-deficient Gram (exact zeros plus rounding noise) with a rank bound
+    # above the rank: threshold-bound, unresolved
     noisy = np.concatenate([np.ones(5), 1e-17 * np.ones(11)])
     assert bond_rank(noisy, 16, 16, None, 1e-15, _EPS_K["fp64"])[0] is None
+    # ... but resolved when the rank bound (avail) is the rank: the values
+    # past it are K's structural null space and set the noise level
+    assert bond_rank(noisy, 16, 5, None, 1e-15, _EPS_K["fp64"])[0] == 5
     # zero, negative or non-finite leading eigenvalue: unresolved
     assert bond_rank(np.zeros(4), 4, 4, 2, 1e-15, _EPS_K["fp64"])[0] is None
     assert bond_rank(np.array([np.nan, 1.0]), 2, 2, 1, 1e-15, _EPS_K["fp64"])[0] is None
 
 
-def test_floor_matches_the_eigensolver_shift():
-    # f = sqrt(4 delta / lambda_0), delta = 8 sqrt(n) eps tr K
+def test_floor_levels():
+    # f = sqrt(FLOOR_C * level / lambda_0), level = max(ASYM_C * asym,
+    # eps sqrt(n) lambda_0, |eigenvalues past avail|)
+    from tncontract_amd.onedim.structured import ASYM_C, FLOOR_C
     lam = np.array([4.0, 1.0, 0.25, 0.0])
+    _, _, f = bond_rank(lam, 4, 4, 2, 1e-15, _EPS_K["fp32"], asym=1e-3)
+    assert abs(f - np.sqrt(FLOOR_C * ASYM_C * 1e-3 / 4.0)) < 1e-15
     _, _, f = bond_rank(lam, 4, 4, 2, 1e-15, _EPS_K["fp32"])
-    delta = 8 * 2.0 * 2.0 ** -23 * 5.25
-    assert abs(f - np.sqrt(4 * delta / 4.0)) < 1e-15
+    assert abs(f - np.sqrt(FLOOR_C * 2.0 ** -23 * 2.0 * 4.0 / 4.0)) < 1e-15
+    _, _, f = bond_rank(lam, 4, 2, 2, 1e-15, _EPS_K["fp32"])
+    assert abs(f - np.sqrt(FLOOR_C * 0.25 / 4.0)) < 1e-15
+
+
+def test_learned_features_and_samples():
+    from tncontract_amd import learned as lr
+    rel = np.exp(-np.arange(40) / 5.0)
+    f = lr.features(rel, 512, 0.25)
+    assert f.shape == (34,) and f.dtype == np.float32
+    np.testing.assert_allclose(f[:32], rel[:32], rtol=1e-6)
+    assert abs(f[32] - np.log(512)) < 1e-6 and abs(f[33] - 0.25) < 1e-7
+    short = lr.features(rel[:5], 16, 0.5)
+    assert np.all(short[5:32] == 0)
+    # the previous sweep's spectrum at bond b pairs with this sweep's kept rank at b
+    X, T = lr.training_samples({1: rel, 2: rel[:10]}, {1: 100, 2: 7, 3: 1}, 128, 4)
+    assert X.shape == (2, 34) and np.allclose(T, [100 / 128, 7 / 128])
+    assert abs(X[0, 33] - 0.25) < 1e-7 and abs(X[1, 33] - 0.5) < 1e-7
+
+
+def test_right_sweep_bond_numbering():
+    # svd_compress's right sweep runs on the reversed chain: its SVD at
+    # reversed position i splits original sites N-2-i | N-1-i, i.e. bond N-1-i
+    from tncontract_amd.learned import _bond_of
+    assert [_bond_of(i, 6) for i in range(5)] == [5, 4, 3, 2, 1]

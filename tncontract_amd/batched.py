@@ -53,7 +53,7 @@ def _bind():
     return lib
 
 
-def batched_truncate(lefts, rights, rel_threshold=1e-8, chi=0, max_sweeps=40):
+def batched_truncate(lefts, rights, rel_threshold=1e-8, chi=0, max_sweeps=60):
     """lefts[j]: (m_j, k_j) FP32 device tensor, rights[j]: (k_j, n_j).
     Returns (U, S, Vt, kept, sweeps): per-job full thin factors (device) and
     host arrays of kept ranks and Jacobi sweeps."""

@@ -66,8 +66,11 @@ __global__ void __launch_bounds__(256) gemm_grouped(const GJob* __restrict__ job
   constexpr int BT = 32 * FR;  // block tile (square)
   constexpr int BK = 16;
   constexpr int LD = BT + 2;
-  __shared__ T As[BK * LD];
-  __shared__ T Bs[BK * LD];
+  constexpr int PER = BT * BK / 256;  // elements of each operand tile per thread
+  // double-buffered: the next k-tile's global loads are in flight in
+  // registers during the current tile's MFMAs; one barrier per k-tile
+  __shared__ T As[2][BK * LD];
+  __shared__ T Bs[2][BK * LD];
   // locate job: largest j with tile_base[j] <= blockIdx.x
   int lo = 0, hi = njobs - 1;
   const int b = blockIdx.x;
@@ -87,24 +90,44 @@ __global__ void __launch_bounds__(256) gemm_grouped(const GJob* __restrict__ job
   for (int i = 0; i < FR; ++i)
 #pragma unroll
     for (int j = 0; j < FR; ++j) acc[i][j] = V4{0, 0, 0, 0};
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    // A tile: lanes walk k (unit stride); B tile: lanes walk n (unit stride)
-    for (int e = tid; e < BT * BK; e += 256) {
-      int kk = e % BK, r = e / BK;
-      int m = m0 + r, k = k0 + kk;
-      As[kk * LD + r] = (T)((m < g.M && k < g.K) ? g.A[(int64_t)m * g.K + k] : 0.f);
-      int c = e % BT, kb = e / BT;
-      int n = n0 + c, k2 = k0 + kb;
-      Bs[kb * LD + c] = (T)((n < g.N && k2 < g.K) ? g.B[(int64_t)k2 * g.N + n] : 0.f);
+  float ra[PER], rb[PER];
+  // A tile: lanes walk k (unit stride); B tile: lanes walk n (unit stride)
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + 256 * i;
+      const int kk = e % BK, r = e / BK;
+      const int m = m0 + r, k = k0 + kk;
+      ra[i] = (m < g.M && k < g.K) ? g.A[(int64_t)m * g.K + k] : 0.f;
+      const int c = e % BT, kb = e / BT;
+      const int n = n0 + c, k2 = k0 + kb;
+      rb[i] = (n < g.N && k2 < g.K) ? g.B[(int64_t)k2 * g.N + n] : 0.f;
     }
-    __syncthreads();
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + 256 * i;
+      As[buf][(e % BK) * LD + e / BK] = (T)ra[i];
+      Bs[buf][(e / BT) * LD + e % BT] = (T)rb[i];
+    }
+  };
+  const int nkt = (g.K + BK - 1) / BK;
+  if (nkt > 0) {
+    fetch(0);
+    stash(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nkt) fetch((kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       T af[FR], bf[FR];
 #pragma unroll
-      for (int i = 0; i < FR; ++i) af[i] = As[(ks + lk) * LD + wr + 16 * i + lr];
+      for (int i = 0; i < FR; ++i) af[i] = As[buf][(ks + lk) * LD + wr + 16 * i + lr];
 #pragma unroll
-      for (int j = 0; j < FR; ++j) bf[j] = Bs[(ks + lk) * LD + wc + 16 * j + lr];
+      for (int j = 0; j < FR; ++j) bf[j] = Bs[buf][(ks + lk) * LD + wc + 16 * j + lr];
 #pragma unroll
       for (int i = 0; i < FR; ++i)
 #pragma unroll
@@ -115,6 +138,7 @@ __global__ void __launch_bounds__(256) gemm_grouped(const GJob* __restrict__ job
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
     }
+    if (kt + 1 < nkt) stash(buf ^ 1);
     __syncthreads();
   }
   // accumulator element e of lane (lr, lk): row 4 lk + e (FP32) or lk + 4 e
@@ -697,9 +721,14 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
     b.nchz = (int)ceil_div(k, BCH);
     b.gram_base = (int)g_job.size();
     b.eig_base = (int)e_job.size();
-    // Orthogonality to L * eps (LAPACK dgesvj's threshold); the products of
-    // the truncation jobs are FP64-accurate (FP64-accumulated FP32 factors).
-    b.tol = (double)L * 2.220446049250313e-16;
+    // Orthogonality to L * eps (LAPACK dgesvj's threshold) for the SVD jobs;
+    // the truncation jobs stop at max(1e-10, L eps): their kept singular
+    // values are then exact to ~1e-20 relative and their vectors to 1e-10,
+    // below the FP32 outputs' rounding, while the clusters of tiny singular
+    // values that the FP32 rounding of the factors leaves (~1e-8 sigma_0)
+    // would need up to twice the sweeps to reach L eps.
+    b.tol = tj ? std::max(1e-10, (double)L * 2.220446049250313e-16)
+               : (double)L * 2.220446049250313e-16;
     b.neg2 = 0.0;
     for (int p = 0; p < b.nblk / 2; ++p) {
       for (int c = 0; c < b.nch; ++c) { g_job.push_back(j); g_pair.push_back(p); g_chunk.push_back(c); }

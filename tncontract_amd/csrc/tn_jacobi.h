@@ -49,6 +49,28 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
   return (unsigned long long)__double_as_longlong(x);
 }
 
+// The pair's Gram entries G_pq, G_pp, G_qq scaled by the same power of two
+// 2^-e (exactly), e the exponent of max(G_pp, G_qq): the rotation test and
+// the convergence measure then square numbers of order one, never the tiny
+// entries of nearly-null vectors, whose squares underflow FP64.
+__device__ __forceinline__ void pair_scaled(double gpq, double gpp, double gqq, double& spq,
+                                            double& spp, double& sqq) {
+  const int e = __builtin_amdgcn_frexp_exp(fmax(gpp, gqq));
+  spq = __builtin_amdgcn_ldexp(gpq, -e);
+  spp = __builtin_amdgcn_ldexp(gpp, -e);
+  sqq = __builtin_amdgcn_ldexp(gqq, -e);
+}
+
+// |G_pq| / sqrt(G_pp G_qq) of one pair, from the scaled entries; 0 where the
+// scaled G_pq^2 underflows (rot_coef_v then makes no rotation either, so the
+// measure never waits on a pair that is not rotated).
+__device__ __forceinline__ double pair_off(double gpq, double gpp, double gqq) {
+  double spq, spp, sqq;
+  pair_scaled(gpq, gpp, gqq, spq, spp, sqq);
+  const double d = spp * sqq;
+  return (spq * spq > 0.0 && d > 0.0) ? fabs(spq) / sqrt(d) : 0.0;
+}
+
 // max_{p<q} |G_pq| / sqrt(G_pp G_qq) over the 32 vectors (block-uniform).
 // Pairs of two negligible vectors (squared norms <= neg2) are left out: the
 // caller has shown that the space they span lies below its truncation
@@ -60,8 +82,7 @@ __device__ inline double eig_off(EigLds& sm, double neg2 = 0.0) {
     int p = e / J2, q = e % J2;
     if (p < q) {
       const double gpp = sm.G[0][p][p], gqq = sm.G[0][q][q];
-      double d = gpp * gqq;
-      if (d > 0.0 && fmax(gpp, gqq) > neg2) off = fmax(off, fabs(sm.G[0][p][q]) / sqrt(d));
+      if (fmax(gpp, gqq) > neg2) off = fmax(off, pair_off(sm.G[0][p][q], gpp, gqq));
     }
   }
   off = wave_max(off);
@@ -88,7 +109,7 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // J_ll = J_hh = c, J_lh = s, J_hl = -s (l < h); row x of J^T G is
 // a*G[x] + b*G[px] with a = c and b = -s (x = l) or +s (x = h).
 // t = tan(angle) = sgn(D P) |P| / (|D| + sqrt(D^2 + P^2)), D = gqq - gpp,
-// P = 2 gpq.  The angle only has to shrink the off-diagonal, so t is
+// P = 2 gpq (the sign from the operands' signs: their product may underflow).  The angle only has to shrink the off-diagonal, so t is
 // evaluated in FP32 on D, P scaled by a power of two (exact) into [0.5, 1):
 // a 2^-24 error in t leaves a 2^-24 fraction of G_lh, which the next sweeps
 // remove like any other (same sweep counts as an FP64 angle, measured).
@@ -99,22 +120,34 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // a branch (a pair that is not rotated selects c = 1, s = 0, discarding
 // whatever the formula gave), so the two rotations a thread needs per round
 // overlap.
-__device__ __forceinline__ void rot_coef(const double (*G)[J2 + 1], int x, int px, double tol,
-                                         double neg2, double& a, double& b) {
-  const int l = min(x, px), h = max(x, px);
-  const double gpq = G[l][h], gpp = G[l][l], gqq = G[h][h];
-  const bool rot = gpq * gpq > tol * tol * fmax(gpp * gqq, 0.0) && fmax(gpp, gqq) > neg2;
+// The coefficients from the three Gram entries of the pair (l, h): G_lh,
+// G_ll, G_hh; x_is_l: whether the index asked for is l.
+__device__ __forceinline__ void rot_coef_v(double gpq, double gpp, double gqq, bool x_is_l,
+                                           double tol, double neg2, double& a, double& b) {
+  double spq, spp, sqq;
+  pair_scaled(gpq, gpp, gqq, spq, spp, sqq);
+  const bool rot = spq * spq > tol * tol * fmax(spp * sqq, 0.0) && fmax(gpp, gqq) > neg2;
   const double D = gqq - gpp, P = 2.0 * gpq;
   const int e = __builtin_amdgcn_frexp_exp(fmax(fabs(D), fabs(P)));
-  const float d = (float)__builtin_amdgcn_ldexp(fabs(D), -e);
-  const float p = (float)__builtin_amdgcn_ldexp(fabs(P), -e);
+  const double dd = __builtin_amdgcn_ldexp(fabs(D), -e), pp = __builtin_amdgcn_ldexp(fabs(P), -e);
+  const float d = (float)dd, p = (float)pp;
   const float hyp = __builtin_amdgcn_sqrtf(fmaf(d, d, p * p));
   const float t32 = p * __builtin_amdgcn_rcpf(d + hyp);
-  const double t = (D * P >= 0.0) ? (double)t32 : -(double)t32;
+  // an angle below FP32's range (|P| < 2^-60 |D|) is t = |P| / (2 |D|) to
+  // FP64's first order: taken from the FP64 reciprocal estimate, so that a
+  // nearly-null vector against a large one still rotates
+  const double tabs = pp < 0x1p-60 ? 0.5 * pp * __builtin_amdgcn_rcp(dd) : (double)t32;
+  const double t = ((D < 0.0) == (P < 0.0)) ? tabs : -tabs;
   const double c = rsq_nr(fma(t, t, 1.0));
   const double sn = t * c;
   a = rot ? c : 1.0;
-  b = rot ? (x == l ? -sn : sn) : 0.0;
+  b = rot ? (x_is_l ? -sn : sn) : 0.0;
+}
+
+__device__ __forceinline__ void rot_coef(const double (*G)[J2 + 1], int x, int px, double tol,
+                                         double neg2, double& a, double& b) {
+  const int l = min(x, px), h = max(x, px);
+  rot_coef_v(G[l][h], G[l][l], G[h][h], x == l, tol, neg2, a, b);
 }
 
 // One cyclic sweep (31 rounds of 16 disjoint rotations, circle order) of the
@@ -136,21 +169,33 @@ __device__ inline int eig_rotate(EigLds& sm, double tol, double neg2 = 0.0) {
   int x0 = rp == 0 ? 0 : rp, x1 = J2 - 1 - rp, y0 = cp == 0 ? 0 : cp, y1 = J2 - 1 - cp;
   for (int r = 0; r < J2 - 1; ++r) {
     const int nxt = cur ^ 1;
+    // every LDS value of the round is read before any arithmetic, so the two
+    // coefficient chains (row pair, column pair) are independent and overlap
+    const int xl = min(x0, x1), xh = max(x0, x1), yl = min(y0, y1), yh = max(y0, y1);
+    double xpq = G[cur][xl][xh], xpp = G[cur][xl][xl], xqq = G[cur][xh][xh];
+    double ypq = G[cur][yl][yh], ypp = G[cur][yl][yl], yqq = G[cur][yh][yh];
+    double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
+    double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
+    double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
+    double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
+    // An empty asm statement that takes and returns every value read: all
+    // fourteen reads are issued together and complete before the coefficient
+    // chains start, instead of the 2x2 block reads being sunk below the chains
+    // (a second LDS latency in every round).  No instruction is emitted.
+    __asm__ volatile("" : "+v"(xpq), "+v"(xpp), "+v"(xqq), "+v"(ypq), "+v"(ypp), "+v"(yqq),
+                     "+v"(g00), "+v"(g01), "+v"(g10), "+v"(g11), "+v"(q00), "+v"(q01),
+                     "+v"(q10), "+v"(q11));
     // (c, b) for the first index of each pair; the partner's b is -b
     double cx, sx, cy, sy;
-    rot_coef(G[cur], x0, x1, tol, neg2, cx, sx);
-    rot_coef(G[cur], y0, y1, tol, neg2, cy, sy);
+    rot_coef_v(xpq, xpp, xqq, x0 == xl, tol, neg2, cx, sx);
+    rot_coef_v(ypq, ypp, yqq, y0 == yl, tol, neg2, cy, sy);
     const double bX0 = sx, bX1 = -sx, bY0 = sy, bY1 = -sy;
-    const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
-    const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
     const double r00 = cx * g00 + bX0 * g10, r01 = cx * g01 + bX0 * g11;
     const double r10 = cx * g10 + bX1 * g00, r11 = cx * g11 + bX1 * g01;
     G[nxt][x0][y0] = cy * r00 + bY0 * r01;
     G[nxt][x0][y1] = cy * r01 + bY1 * r00;
     G[nxt][x1][y0] = cy * r10 + bY0 * r11;
     G[nxt][x1][y1] = cy * r11 + bY1 * r10;
-    const double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
-    const double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
     Q[nxt][x0][y0] = cy * q00 + bY0 * q01;
     Q[nxt][x0][y1] = cy * q01 + bY1 * q00;
     Q[nxt][x1][y0] = cy * q10 + bY0 * q11;

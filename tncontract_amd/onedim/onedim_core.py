@@ -147,8 +147,11 @@ class MatrixProductState(OneDimensionalTensorNetwork):
                                                       self.right_label])
 
     def left_canonise(self, start=0, end=-1, chi=None, threshold=1e-14, normalise=False,
-                      qr_decomposition=False):
-        """onedim_core.py:218-358"""
+                      qr_decomposition=False, _svd=None):
+        """onedim_core.py:218-358.  `_svd` (internal): replaces the site SVD of
+        the SVD path, called as _svd(site, row_labels, svd_label, i) and
+        returning what tensor._tensor_svd returns, with at least as many
+        singular values as the keep rule selects."""
         N = len(self)
         if end == -1:
             end = N
@@ -182,8 +185,11 @@ class MatrixProductState(OneDimensionalTensorNetwork):
                     self[i].scale_(norm)
                 return
             svd_label = unique_label()
-            U, S, V, s = tsr._tensor_svd(self[i], [self.phys_label, self.left_label],
-                                         svd_label=svd_label)
+            if _svd is None:
+                U, S, V, s = tsr._tensor_svd(self[i], [self.phys_label, self.left_label],
+                                             svd_label=svd_label)
+            else:
+                U, S, V, s = _svd(self[i], [self.phys_label, self.left_label], svd_label, i)
             sv = s.cpu().numpy()
             s0 = sv[0] if len(sv) else 0.0
             if s0 == 0.0:
@@ -208,14 +214,15 @@ class MatrixProductState(OneDimensionalTensorNetwork):
                 self[i + 1].scale_(norm)
 
     def right_canonise(self, start=0, end=-1, chi=None, threshold=1e-14, normalise=False,
-                       qr_decomposition=False):
-        """onedim_core.py:360-378"""
+                       qr_decomposition=False, _svd=None):
+        """onedim_core.py:360-378 (`_svd`: as in left_canonise, on the reversed
+        chain)"""
         self.reverse()
         N = len(self)
         if end == -1:
             end = N
         self.left_canonise(start=N - end, end=N - start, chi=chi, threshold=threshold,
-                           normalise=normalise, qr_decomposition=qr_decomposition)
+                           normalise=normalise, qr_decomposition=qr_decomposition, _svd=_svd)
         self.reverse()
 
     def norm(self, canonical_form=False):
@@ -226,9 +233,10 @@ class MatrixProductState(OneDimensionalTensorNetwork):
             return self[0].norm()
         return float(np.sqrt(inner_product_mps(self, self)))
 
-    def svd_compress(self, chi=None, threshold=1e-15, normalise=False, reverse=False):
+    def svd_compress(self, chi=None, threshold=1e-15, normalise=False, reverse=False, _svd=None):
         """onedim_core.py:463-484: QR left sweep, normalise, truncating SVD
-        right sweep, restore the norm."""
+        right sweep, restore the norm.  `_svd` (internal): the right sweep's
+        site SVD, as in right_canonise."""
         if reverse:
             self.reverse()
         self.left_canonise(normalise=False, qr_decomposition=True)
@@ -241,7 +249,7 @@ class MatrixProductState(OneDimensionalTensorNetwork):
             self[-1].scale_(float("nan"))
         else:
             self[-1].scale_(1.0 / norm)
-        self.right_canonise(chi=chi, threshold=threshold, normalise=False)
+        self.right_canonise(chi=chi, threshold=threshold, normalise=False, _svd=_svd)
         if not normalise:
             self[0].scale_(norm)
         if reverse:

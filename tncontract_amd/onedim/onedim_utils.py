@@ -9,7 +9,8 @@ from tncontract_amd import tensor as tnc
 from tncontract_amd.onedim import onedim_core as onedim
 
 __all__ = ["init_mps_random", "init_mps_allzero", "init_mps_logical", "onebody_sum_mpo",
-           "expvals_mps", "xxz_mpo", "xxz_bulk_w", "random_mps_arrays", "random_mps"]
+           "expvals_mps", "xxz_mpo", "xxz_bulk_w", "random_mps_arrays", "random_mps",
+           "decaying_mps_arrays"]
 
 
 def init_mps_random(nsites, physdim, bonddim=1, left_label="left", right_label="right",
@@ -196,3 +197,23 @@ def random_mps(N, d, chi, seed, phys_label="phys", left_label="left", right_labe
     arrs = random_mps_arrays(N, d, chi, seed)
     return onedim.MatrixProductState([tnc.Tensor(a, [phys_label, left_label, right_label])
                                       for a in arrs], left_label, right_label, phys_label)
+
+
+def decaying_mps_arrays(N, d, chi, taus, seed, dtype=np.float32):
+    """Seeded MPS site arrays (phys, left, right) with exponentially decaying
+    bond spectra: bonds b_i = min(d^i, d^(N-i), chi); site i is a random right
+    isometry (rows of a QR of a Gaussian, b_i x d b_{i+1}) whose left index k
+    is scaled by exp(-k / taus[i - 1]) (site 0 unscaled).  The Schmidt values
+    at bond i then fall off roughly like exp(-k / taus[i - 1]).  Returned in
+    `dtype` (the learned-rank workload's data are FP32)."""
+    rng = np.random.default_rng(seed)
+    b = [min(d ** i, d ** (N - i), chi) for i in range(N + 1)]
+    arrs = []
+    for i in range(N):
+        l, r = b[i], b[i + 1]
+        q, _ = np.linalg.qr(rng.standard_normal((d * r, l)))     # (d r) x l, orthonormal columns
+        site = q.T.reshape(l, d, r).transpose(1, 0, 2)           # (phys, left, right)
+        if i > 0:
+            site = site * np.exp(-np.arange(l) / float(taus[i - 1]))[None, :, None]
+        arrs.append(np.ascontiguousarray(site).astype(dtype))
+    return arrs

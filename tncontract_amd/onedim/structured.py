@@ -28,18 +28,19 @@ bond then depends on the left part through its Gram matrix alone.  So:
    Y_kept (right-isometric, labels [physout, right, left]);  P_i = C_i Y_kept.
 3. Site 0 = A_0 P_1 (labels [right, physout, left]).
 
-Resolution: K holds sigma^2, so a bond's eigenvalues are only resolved down
-to the eigensolver's Cholesky shift delta = 8 sqrt(n) eps_K tr K (eps_K the
-machine epsilon of the precision K was formed in); below sigma / sigma_0 =
-f = sqrt(4 delta / lambda_0) a value cannot be told from rounding (f ~ 1e-6
-for an FP64 K at n = 1024, ~1e-1 for an FP32 one).  A bond's rank is
-therefore taken here only when it is decided above f: by a threshold >= f,
-or by the chi / QR-bond cap binding above f.  A bond whose rank would be
-decided below f (a threshold-bound truncation of a decaying or rank-deficient
-spectrum, a chi cut deep in the tail, a zero state) sends the whole call to
-the exact path, contract_mps_mpo + svd_compress, so the result always has the
-reference's bond dimensions; ``SweepStats.fallback`` records the bond that
-did.
+Resolution: K holds sigma^2, so a bond's eigenvalues are resolved only down
+to K's rounding level: ASYM_C ||K - K^T||_F (the asymmetry of the formed K
+samples the rounding of its two triangles), or the eigenvalues of K's
+structural null space where the bond's rank bound leaves one, and at least
+eps sqrt(n) lambda_0.  Below sigma / sigma_0 = f = sqrt(FLOOR_C level /
+lambda_0) a value cannot be told from rounding (f ~ 1e-7 for an FP64 K,
+~1e-3 .. 1e-2 for an FP32 one).  A bond's rank is therefore taken here only
+when it is decided above f: by a threshold >= f, or by the chi / QR-bond cap
+binding above f.  A bond whose rank would be decided below f (a threshold-
+bound truncation of a decaying or rank-deficient spectrum, a chi cut deep in
+the tail, a zero state) sends the whole call to the exact path,
+contract_mps_mpo + svd_compress, so the result always has the reference's
+bond dimensions; ``SweepStats.fallback`` records the bond that did.
 
 Precision: "fp64" runs every contraction in FP64 (the algorithmic change
 alone); "fp32" / "bf16x3" run the contractions on FP32 / split-bf16 MFMA with
@@ -105,9 +106,14 @@ MIXED_EIG_TOL = 1e-10
 
 
 # machine epsilon of K by the precision the contractions run in (bf16x3
-# products carry FP32-level rounding); the eigensolver's shift uses the same
+# products carry FP32-level rounding)
 _EPS_K = {"fp64": 2.0 ** -52, "native": 2.0 ** -52, "fp32": 2.0 ** -23, "bf16x3": 2.0 ** -23}
-# eigenvalues below FLOOR_C * delta are taken as unresolved
+# the rounding of K, estimated from its asymmetry ||K - K^T||_F (the two
+# triangles are separate sums; the environment's own rounding, which is
+# symmetric, is covered by the factor ASYM_C), or observed directly in the
+# eigenvalues of K's structural null space; eigenvalues below FLOOR_C times
+# that level are taken as unresolved
+ASYM_C = 8.0
 FLOOR_C = 4.0
 
 
@@ -119,19 +125,26 @@ class _Unresolved(Exception):
         self.bond = bond
 
 
-def bond_rank(lam, n, avail, chi, threshold, eps_k):
+def bond_rank(lam, n, avail, chi, threshold, eps_k, asym=0.0):
     """Kept rank at a bond from the eigenvalues lam (descending, host) of its
-    Gram matrix, by the reference's rule (sigma / sigma_0 > threshold, then at
-    most chi; at most `avail`, the bond dimension the reference's QR sweep
-    gives), or None when that rank is decided below the resolution floor.
-    Returns (keep or None, rel sigma / sigma_0 over avail, floor f)."""
+    Gram matrix K, by the reference's rule (sigma / sigma_0 > threshold, then
+    at most chi; at most `avail`, the bond dimension the reference's QR sweep
+    gives, which bounds the rank of K), or None when that rank is decided
+    below the resolution floor.  asym: ||K - K^T||_F of the K the eigensolver
+    symmetrised.  The rounding level of K is the largest of ASYM_C * asym,
+    the magnitude of the eigenvalues past `avail` (zero in exact arithmetic)
+    and eps_k sqrt(n) lambda_0; the floor is f = sqrt(FLOOR_C level /
+    lambda_0) in sigma / sigma_0.  Returns (keep or None, rel sigma /
+    sigma_0 over avail, f)."""
     lam = np.asarray(lam, dtype=np.float64)
     if lam.size == 0 or not np.isfinite(lam[0]) or lam[0] <= 0.0 or not np.all(np.isfinite(lam)):
         return None, np.zeros(0), float("inf")
-    pos = np.maximum(lam, 0.0)
-    delta = 8.0 * np.sqrt(n) * eps_k * float(np.sum(pos))
-    f = float(np.sqrt(FLOOR_C * delta / lam[0]))
-    rel = np.sqrt(pos[:avail] / lam[0])
+    lam0 = float(lam[0])
+    level = max(ASYM_C * float(asym), eps_k * np.sqrt(n) * lam0)
+    if avail < lam.size:
+        level = max(level, float(np.max(np.abs(lam[avail:]))))
+    f = float(np.sqrt(FLOOR_C * level / lam0))
+    rel = np.sqrt(np.maximum(lam[:avail], 0.0) / lam0)
     cap = min(chi, avail) if chi else avail
     if threshold >= f:
         return min(int(np.count_nonzero(rel > threshold)), cap), rel, f
@@ -211,7 +224,9 @@ def _apply_mpo_compress(mps, mpo, chi, threshold, precision, stats, eig_tol):
             raise _Unresolved(i)
         lam_h = lam.cpu().numpy()
         avail = min(n, qb[i])
-        keep, rel, f = bond_rank(lam_h, n, avail, chi, threshold, _EPS_K[precision])
+        # ||K - K^T||_F: K is the (t', k') x (t, k) view, K^T swaps the groups
+        asym = float(np.sqrt(L.sumsq(L.axpby(Km, Km.permute(2, 3, 0, 1), 1.0, -1.0))))
+        keep, rel, f = bond_rank(lam_h, n, avail, chi, threshold, _EPS_K[precision], asym)
         if stats is not None:
             stats.spectra[i] = rel
             stats.avail[i] = avail
