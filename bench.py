@@ -714,7 +714,7 @@ def learned_workload(args):
            "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64 (FP32 data)",
            "data": "synthetic (seeded d=4 MPS, input bond 1024, bond spectra ~exp(-k/tau), "
-                   "tau ~ U[4,64])",
+                   "tau ~ U[2,16])",
            "config": {"workload": "svd_compress to chi=512 at sigma/sigma_0 > 1e-8, N=16, d=4, "
                                   "input bond 1024 (bond matrices up to 2048 x 1024); rank model "
                                   "trained on %d sweep samples" % sum(len(x[1]) for x in samples)},

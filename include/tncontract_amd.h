@@ -167,7 +167,12 @@ int tn_cholqr(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int
  * not reach orthogonality 1e-13) and the Jacobi runs on the rows of R.
  * tol: orthogonality tolerance (<= 0 -> (vector length) * eps, as LAPACK
  * dgesvj); max_sweeps <= 0 -> 40.  *sweeps_out (host, may be NULL) receives
- * the number of sweeps used.  Returns TN_ERR_NOCONV if not converged. */
+ * the number of sweeps used.  Returns TN_ERR_NOCONV if not converged.
+ * A singular value that is exactly zero gets a zero column of U or a zero
+ * row of Vt (in whichever factor is formed by dividing rotated vectors by
+ * their norms), where LAPACK returns an orthonormal completion; the
+ * reference's callers truncate such values away or multiply them by zero, so
+ * the product U S Vt is the same. */
 size_t tn_svd_workspace(int64_t m, int64_t n, int dtype);
 int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m, int64_t n,
            void* U, double* S, void* Vt, int dtype, double tol, int max_sweeps, int* sweeps_out,

@@ -178,13 +178,6 @@ __device__ inline int eig_rotate(EigLds& sm, double tol, double neg2 = 0.0) {
     double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
     double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
     double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
-    // An empty asm statement that takes and returns every value read: all
-    // fourteen reads are issued together and complete before the coefficient
-    // chains start, instead of the 2x2 block reads being sunk below the chains
-    // (a second LDS latency in every round).  No instruction is emitted.
-    __asm__ volatile("" : "+v"(xpq), "+v"(xpp), "+v"(xqq), "+v"(ypq), "+v"(ypp), "+v"(yqq),
-                     "+v"(g00), "+v"(g01), "+v"(g10), "+v"(g11), "+v"(q00), "+v"(q01),
-                     "+v"(q10), "+v"(q11));
     // (c, b) for the first index of each pair; the partner's b is -b
     double cx, sx, cy, sy;
     rot_coef_v(xpq, xpp, xqq, x0 == xl, tol, neg2, cx, sx);

@@ -257,12 +257,14 @@ class LearnedSweepStats:
         return sum(1 for p in self.path.values() if p == "fallback")
 
 
-def learned_svd_compress(mps, chi, threshold, model, prev_spectra, stats=None, margin=8,
-                         oversample=10, power_iters=2, seed=0):
+def learned_svd_compress(mps, chi, threshold, model, prev_spectra, stats=None, margin=16,
+                         scale=1.2, oversample=10, power_iters=2, seed=0):
     """svd_compress(chi, threshold) (onedim_core.py:463-484) of a copy of
     `mps` with each truncating SVD of the right sweep done by certified_svd
     at the rank `model` predicts from prev_spectra[bond] (sigma / sigma_0 of
-    the previous sweep at that bond; bonds without one use chi).  The keep
+    the previous sweep at that bond; bonds without one use chi), widened to
+    scale * rank + margin: an under-predicted rank costs a sketch and the
+    full SVD, an over-predicted one only a wider sketch.  The keep
     rule, the step order and the outputs are svd_compress's; a bond whose
     sketch is not certified falls back to the full Jacobi SVD.  Returns the
     compressed MPS."""
@@ -281,7 +283,7 @@ def learned_svd_compress(mps, chi, threshold, model, prev_spectra, stats=None, m
         shape = t.shape
         m = int(np.prod(shape[:nrow]))
         A = L.materialize(t.data).view(m, -1)
-        r = ranks.get(b, chi) + margin
+        r = int(np.ceil(scale * ranks.get(b, chi))) + margin
         U, s, vt, info = certified_svd(A, r, threshold, oversample, power_iters, seed + b)
         k = s.numel()
         if stats is not None:
@@ -358,10 +360,13 @@ def _state(arrs, dtype=torch.float64):
                                           ["phys", "left", "right"]) for a in arrs])
 
 
-def decaying_pair(seed, N=16, d=4, chi_in=1024, tau_range=(4.0, 64.0)):
+def decaying_pair(seed, N=16, d=4, chi_in=1024, tau_range=(2.0, 16.0)):
     """Two seeded states of the learned-rank workload with the same bond decay
     rates (one tau ~ U[tau_range] per bond) and different isometries: the
-    previous sweep's state and the current one.  FP32 data (host arrays)."""
+    previous sweep's state and the current one.  FP32 data (host arrays).
+    The default rates are those of fast-decaying (gapped, ground-state-like)
+    spectra, whose truncation at 1e-8 keeps ~18 tau of the 1024 values: the
+    regime a rank-sized sketch is for."""
     from tncontract_amd.onedim.onedim_utils import decaying_mps_arrays
     rng = np.random.default_rng(seed)
     taus = rng.uniform(tau_range[0], tau_range[1], size=N - 1)

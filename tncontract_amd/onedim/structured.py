@@ -29,10 +29,11 @@ bond then depends on the left part through its Gram matrix alone.  So:
 3. Site 0 = A_0 P_1 (labels [right, physout, left]).
 
 Resolution: K holds sigma^2, so a bond's eigenvalues are resolved only down
-to K's rounding level: ASYM_C ||K - K^T||_F (the asymmetry of the formed K
-samples the rounding of its two triangles), or the eigenvalues of K's
-structural null space where the bond's rank bound leaves one, and at least
-eps sqrt(n) lambda_0.  Below sigma / sigma_0 = f = sqrt(FLOOR_C level /
+to K's rounding level: ASYM_C 2 ||K - K^T||_F / sqrt(n) (the asymmetry of
+the formed K samples the rounding of its two triangles, and rounding is
+close to random noise, whose spectral norm is ~2 / sqrt(n) of its Frobenius
+norm), or the eigenvalues of K's structural null space where the bond's rank
+bound leaves one, and at least eps sqrt(n) lambda_0.  Below sigma / sigma_0 = f = sqrt(FLOOR_C level /
 lambda_0) a value cannot be told from rounding (f ~ 1e-7 for an FP64 K,
 ~1e-3 .. 1e-2 for an FP32 one).  A bond's rank is therefore taken here only
 when it is decided above f: by a threshold >= f, or by the chi / QR-bond cap
@@ -44,8 +45,9 @@ bond dimensions; ``SweepStats.fallback`` records the bond that did.
 
 Precision: "fp64" runs every contraction in FP64 (the algorithmic change
 alone); "fp32" / "bf16x3" run the contractions on FP32 / split-bf16 MFMA with
-FP32 accumulation, while K, the Jacobi rotations and the output isometries
-stay FP64 (the eigensolver then stops at orthogonality 1e-10 instead of
+FP32 accumulation (bf16x3: each FP32 operand split into two bf16 halves,
+three products, ~2^-16 relative rounding per product), while K, the Jacobi
+rotations and the output isometries stay FP64 (the eigensolver then stops at orthogonality 1e-10 instead of
 n * eps: K formed in FP32 carries ~1e-7 relative rounding anyway).  Flop
 count per bulk site at chi=512, D=5, d=2: ~52 GFLOP versus ~250 GFLOP for
 the reference's QR + SVD on the 2560-bond chain.
@@ -105,9 +107,10 @@ def _cdt(x, dtype):
 MIXED_EIG_TOL = 1e-10
 
 
-# machine epsilon of K by the precision the contractions run in (bf16x3
-# products carry FP32-level rounding)
-_EPS_K = {"fp64": 2.0 ** -52, "native": 2.0 ** -52, "fp32": 2.0 ** -23, "bf16x3": 2.0 ** -23}
+# relative rounding of one product by the precision the contractions run in:
+# bf16x3 splits each FP32 operand into two bf16 halves (16 significant bits)
+# and drops the lo * lo product, ~2^-16 of |x y|
+_EPS_K = {"fp64": 2.0 ** -52, "native": 2.0 ** -52, "fp32": 2.0 ** -23, "bf16x3": 2.0 ** -16}
 # the rounding of K, estimated from its asymmetry ||K - K^T||_F (the two
 # triangles are separate sums; the environment's own rounding, which is
 # symmetric, is covered by the factor ASYM_C), or observed directly in the
@@ -131,16 +134,18 @@ def bond_rank(lam, n, avail, chi, threshold, eps_k, asym=0.0):
     at most chi; at most `avail`, the bond dimension the reference's QR sweep
     gives, which bounds the rank of K), or None when that rank is decided
     below the resolution floor.  asym: ||K - K^T||_F of the K the eigensolver
-    symmetrised.  The rounding level of K is the largest of ASYM_C * asym,
-    the magnitude of the eigenvalues past `avail` (zero in exact arithmetic)
-    and eps_k sqrt(n) lambda_0; the floor is f = sqrt(FLOOR_C level /
-    lambda_0) in sigma / sigma_0.  Returns (keep or None, rel sigma /
-    sigma_0 over avail, f)."""
+    symmetrised.  The rounding level of K (a spectral norm) is the largest
+    of ASYM_C * 2 asym / sqrt(n) (rounding errors are close to random, and a
+    random n x n matrix has spectral norm ~2 / sqrt(n) of its Frobenius
+    norm), the magnitude of the eigenvalues past `avail` (zero in exact
+    arithmetic) and eps_k sqrt(n) lambda_0; the floor is f = sqrt(FLOOR_C
+    level / lambda_0) in sigma / sigma_0.  Returns (keep or None, rel sigma
+    / sigma_0 over avail, f)."""
     lam = np.asarray(lam, dtype=np.float64)
     if lam.size == 0 or not np.isfinite(lam[0]) or lam[0] <= 0.0 or not np.all(np.isfinite(lam)):
         return None, np.zeros(0), float("inf")
     lam0 = float(lam[0])
-    level = max(ASYM_C * float(asym), eps_k * np.sqrt(n) * lam0)
+    level = max(ASYM_C * 2.0 * float(asym) / np.sqrt(n), eps_k * np.sqrt(n) * lam0)
     if avail < lam.size:
         level = max(level, float(np.max(np.abs(lam[avail:]))))
     f = float(np.sqrt(FLOOR_C * level / lam0))
