@@ -109,17 +109,21 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // J_ll = J_hh = c, J_lh = s, J_hl = -s (l < h); row x of J^T G is
 // a*G[x] + b*G[px] with a = c and b = -s (x = l) or +s (x = h).
 // t = tan(angle) = sgn(D P) |P| / (|D| + sqrt(D^2 + P^2)), D = gqq - gpp,
-// P = 2 gpq (the sign from the operands' signs: their product may underflow).  The angle only has to shrink the off-diagonal, so t is
-// evaluated in FP32 on D, P scaled by a power of two (exact) into [0.5, 1):
-// a 2^-24 error in t leaves a 2^-24 fraction of G_lh, which the next sweeps
-// remove like any other (same sweep counts as an FP64 angle, measured).
-// Orthogonality comes from c = 1/sqrt(1 + t^2) in FP64 (v_rsq_f64 + two
-// Newton steps) and s = t c, so c^2 + s^2 = 1 to FP64 rounding.  A pair is
-// rotated when gpq^2 > tol^2 gpp gqq and one of its vectors is not
-// negligible (squared norm > neg2; see eig_off); the coefficients are computed without
-// a branch (a pair that is not rotated selects c = 1, s = 0, discarding
-// whatever the formula gave), so the two rotations a thread needs per round
-// overlap.
+// P = 2 gpq (the sign from the operands' signs: their product may
+// underflow).  The angle only has to shrink the off-diagonal, so
+// 1 / (|D| + sqrt(D^2 + P^2)) is evaluated in FP32 on D, P scaled by a power
+// of two (exact) into [0.5, 1), where it lies in (0.5, 2]: a 2^-24 error in
+// t leaves a 2^-24 fraction of G_lh, which the next sweeps remove like any
+// other (same sweep counts as an FP64 angle, measured).  t is that
+// reciprocal times the scaled |P| in FP64, so an angle far below FP32's
+// range (a nearly-null vector against a large one) is still |P| / (2 |D|)
+// and rotates.  Orthogonality comes from c = 1/sqrt(1 + t^2) in FP64
+// (v_rsq_f64 + two Newton steps) and s = t c, so c^2 + s^2 = 1 to FP64
+// rounding.  A pair is rotated when gpq^2 > tol^2 gpp gqq (on the entries
+// scaled by pair_scaled) and one of its vectors is not negligible (squared
+// norm > neg2; see eig_off); the coefficients are computed without a branch
+// (a pair that is not rotated selects c = 1, s = 0, discarding whatever the
+// formula gave).
 // The coefficients from the three Gram entries of the pair (l, h): G_lh,
 // G_ll, G_hh; x_is_l: whether the index asked for is l.
 __device__ __forceinline__ void rot_coef_v(double gpq, double gpp, double gqq, bool x_is_l,
@@ -132,11 +136,7 @@ __device__ __forceinline__ void rot_coef_v(double gpq, double gpp, double gqq, b
   const double dd = __builtin_amdgcn_ldexp(fabs(D), -e), pp = __builtin_amdgcn_ldexp(fabs(P), -e);
   const float d = (float)dd, p = (float)pp;
   const float hyp = __builtin_amdgcn_sqrtf(fmaf(d, d, p * p));
-  const float t32 = p * __builtin_amdgcn_rcpf(d + hyp);
-  // an angle below FP32's range (|P| < 2^-60 |D|) is t = |P| / (2 |D|) to
-  // FP64's first order: taken from the FP64 reciprocal estimate, so that a
-  // nearly-null vector against a large one still rotates
-  const double tabs = pp < 0x1p-60 ? 0.5 * pp * __builtin_amdgcn_rcp(dd) : (double)t32;
+  const double tabs = pp * (double)__builtin_amdgcn_rcpf(d + hyp);
   const double t = ((D < 0.0) == (P < 0.0)) ? tabs : -tabs;
   const double c = rsq_nr(fma(t, t, 1.0));
   const double sn = t * c;
@@ -144,44 +144,45 @@ __device__ __forceinline__ void rot_coef_v(double gpq, double gpp, double gqq, b
   b = rot ? (x_is_l ? -sn : sn) : 0.0;
 }
 
-__device__ __forceinline__ void rot_coef(const double (*G)[J2 + 1], int x, int px, double tol,
-                                         double neg2, double& a, double& b) {
-  const int l = min(x, px), h = max(x, px);
-  rot_coef_v(G[l][h], G[l][l], G[h][h], x == l, tol, neg2, a, b);
+// v from the lane whose byte address (4 * lane) is addr, within the wave.
+__device__ __forceinline__ double lane_fetch(int addr, double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)u);
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(u >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // One cyclic sweep (31 rounds of 16 disjoint rotations, circle order) of the
 // two-sided Jacobi on G, accumulated into Q, one barrier per round: thread t
 // owns the 2x2 block (row pair t/16, column pair t%16) of G' = J^T G J and
-// Q' = Q J and computes the two rotations it needs itself (the 16 threads
-// that need a rotation compute it from the same LDS values, so they agree
-// bit for bit).  One inner sweep per outer step: the outer block iteration
-// needs the same number of sweeps with one inner sweep as with full inner
-// convergence (measured), at a fraction of the cost.  Returns the buffer
-// holding Q.
+// Q' = Q J.  Each lane computes the rotation of its column pair from the LDS
+// values; the rotation of its row pair is the one computed by the lane of
+// the same 16-lane row whose column pair it is (fetched across the wave, so
+// row and column rotations of a pair are the same bits).  One inner sweep
+// per outer step: the outer block iteration needs the same number of sweeps
+// with one inner sweep as with full inner convergence (measured), at a
+// fraction of the cost.  Returns the buffer holding Q.
 __device__ inline int eig_rotate(EigLds& sm, double tol, double neg2 = 0.0) {
   auto& G = sm.G;
   auto& Q = sm.Q;
   const int tid = threadIdx.x;
   const int rp = tid >> 4, cp = tid & 15;
+  const int src = ((tid & 0x30) | rp) << 2;   // lane of this row with column pair rp
   int cur = 0;
   // circle positions, advanced by one (mod 31, in 1..31) each round
   int x0 = rp == 0 ? 0 : rp, x1 = J2 - 1 - rp, y0 = cp == 0 ? 0 : cp, y1 = J2 - 1 - cp;
   for (int r = 0; r < J2 - 1; ++r) {
     const int nxt = cur ^ 1;
-    // every LDS value of the round is read before any arithmetic, so the two
-    // coefficient chains (row pair, column pair) are independent and overlap
-    const int xl = min(x0, x1), xh = max(x0, x1), yl = min(y0, y1), yh = max(y0, y1);
-    double xpq = G[cur][xl][xh], xpp = G[cur][xl][xl], xqq = G[cur][xh][xh];
-    double ypq = G[cur][yl][yh], ypp = G[cur][yl][yl], yqq = G[cur][yh][yh];
-    double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
-    double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
-    double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
-    double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
+    const int yl = min(y0, y1), yh = max(y0, y1);
+    const double ypq = G[cur][yl][yh], ypp = G[cur][yl][yl], yqq = G[cur][yh][yh];
+    const double g00 = G[cur][x0][y0], g01 = G[cur][x0][y1];
+    const double g10 = G[cur][x1][y0], g11 = G[cur][x1][y1];
+    const double q00 = Q[cur][x0][y0], q01 = Q[cur][x0][y1];
+    const double q10 = Q[cur][x1][y0], q11 = Q[cur][x1][y1];
     // (c, b) for the first index of each pair; the partner's b is -b
-    double cx, sx, cy, sy;
-    rot_coef_v(xpq, xpp, xqq, x0 == xl, tol, neg2, cx, sx);
+    double cy, sy;
     rot_coef_v(ypq, ypp, yqq, y0 == yl, tol, neg2, cy, sy);
+    const double cx = lane_fetch(src, cy), sx = lane_fetch(src, sy);
     const double bX0 = sx, bX1 = -sx, bY0 = sy, bY1 = -sy;
     const double r00 = cx * g00 + bX0 * g10, r01 = cx * g01 + bX0 * g11;
     const double r10 = cx * g10 + bX1 * g00, r11 = cx * g11 + bX1 * g01;
