@@ -178,6 +178,22 @@ int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_
            void* U, double* S, void* Vt, int dtype, double tol, int max_sweeps, int* sweeps_out,
            void* ws, size_t ws_bytes, void* stream);
 
+/* ---- allocation-free SVD ------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (serving loops and harnesses that forbid hipMalloc).  The same
+ * computation and results as tn_svd, with the workspace taken from a
+ * library-owned scratch array of TN_SVD_NOALLOC_SCRATCH_BYTES per GPU (held
+ * until the process ends) instead of a caller's buffer; it calls no
+ * allocating routine and does not replay its sweeps as a HIP graph.  An SVD
+ * whose tn_svd_workspace exceeds the scratch size is refused with
+ * TN_ERR_WORKSPACE before any device work.  Every call on one GPU uses that
+ * GPU's one array: calls that overlap on the same GPU (other streams or host
+ * threads) race and must be serialised by the caller. */
+#define TN_SVD_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tn_svd_noalloc(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m,
+                   int64_t n, void* U, double* S, void* Vt, int dtype, double tol,
+                   int max_sweeps, int* sweeps_out, void* stream);
+
 /* ---- PSD eigendecomposition (truncation from a bond Gram matrix) ---------
  * K = U diag(lam) U^T for a symmetric positive semi-definite n x n matrix K
  * (decoded as in tn_qr; FP64 or FP32 input).  U: n x n row-major FP64

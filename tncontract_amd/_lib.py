@@ -96,6 +96,8 @@ def lib():
         "tn_svd_workspace": (sz, [i64, i64, i32]),
         "tn_svd": (i32, [vp, mp, mp, i64, i64, vp, vp, vp, i32, dbl, i32,
                          ctypes.POINTER(ctypes.c_int), vp, sz, vp]),
+        "tn_svd_noalloc": (i32, [vp, mp, mp, i64, i64, vp, vp, vp, i32, dbl, i32,
+                                 ctypes.POINTER(ctypes.c_int), vp]),
         "tn_eigh_psd_workspace": (sz, [i64]),
         "tn_eigh_psd": (i32, [vp, mp, mp, i64, vp, vp, i32, dbl, i32,
                               ctypes.POINTER(ctypes.c_int), vp, sz, vp]),

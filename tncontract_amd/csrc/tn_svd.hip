@@ -26,6 +26,7 @@
 
 
 #include <cmath>
+#include <mutex>
 #include <cstring>
 #include <vector>
 
@@ -327,7 +328,8 @@ static size_t svd_layout(int64_t m, int64_t n, char* base, size_t cap, SvdWs* ou
 // One-sided block Jacobi on k vectors of length L (Wv, vector-contiguous),
 // accumulating the rotations into Z (k x k, vector-contiguous).
 static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& w, double tol,
-                      int max_sweeps, int* sweeps_out, bool* converged, hipStream_t st) {
+                      int max_sweeps, int* sweeps_out, bool* converged, hipStream_t st,
+                      bool allow_graph = true) {
   int nblk = (int)ceil_div(std::max<int64_t>(k, 1), JB);
   if (nblk % 2) nblk += 1;
   const int npairs = nblk / 2;
@@ -371,7 +373,7 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
   // graph and replay it (long sweeps only; not while launch timing is on).
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
-  if (nblk - 1 >= 16 && !prof_on()) {
+  if (allow_graph && nblk - 1 >= 16 && !prof_on()) {
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
       int rc = enqueue_sweep();
       hipError_t e = hipStreamEndCapture(st, &graph);
@@ -430,9 +432,14 @@ extern "C" size_t tn_svd_workspace(int64_t m, int64_t n, int dtype) {
   return svd_layout(m, n, nullptr, 0, nullptr) + 256;
 }
 
-extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m,
-                      int64_t n, void* U, double* S, void* Vt, int dtype, double tol,
-                      int max_sweeps, int* sweeps_out, void* ws, size_t ws_bytes, void* stream) {
+namespace tn {
+
+// tn_svd with the graph replay of the Jacobi sweeps optional (tn_svd_noalloc
+// leaves it off: instantiating a graph is not free of allocations).
+static int svd_impl(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m,
+                    int64_t n, void* U, double* S, void* Vt, int dtype, double tol,
+                    int max_sweeps, int* sweeps_out, void* ws, size_t ws_bytes, void* stream,
+                    bool allow_graph) {
   TN_REQUIRE(m >= 0 && n >= 0, "tn_svd: negative size");
   TN_REQUIRE(dtype == TN_F64 || dtype == TN_F32, "tn_svd: unsupported dtype %d", dtype);
   Modes rows = to_modes(a_rows), cols = to_modes(a_cols);
@@ -492,7 +499,8 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
       TN_CHECK_LAUNCH();
     }
     if (tol <= 0.0) tol = std::max(1e-15, (double)k * 2.220446049250313e-16);
-    TN_TRY(jacobi_run(w.R, k, w.Z, k, w, tol, max_sweeps, sweeps_out, &converged, st));
+    TN_TRY(jacobi_run(w.R, k, w.Z, k, w, tol, max_sweeps, sweeps_out, &converged, st,
+                      allow_graph));
     double* QZv;  // Q Z (L x k, vector-contiguous)
     if (householder) {
       TN_TRY(qr_form_q(w.QZ, L, k, w.qr, w.Wv, st, w.Z));
@@ -512,7 +520,8 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
   } else {
     // LAPACK dgesvj convergence threshold: (vector length) * eps
     if (tol <= 0.0) tol = std::max(1e-15, (double)L * 2.220446049250313e-16);
-    TN_TRY(jacobi_run(w.Wv, L, w.Z, k, w, tol, max_sweeps, sweeps_out, &converged, st));
+    TN_TRY(jacobi_run(w.Wv, L, w.Z, k, w, tol, max_sweeps, sweeps_out, &converged, st,
+                      allow_graph));
     orth_src = w.Wv;
     orth_len = L;
     if (tall) { Uvec = w.Wv; ldu = L; u_scaled = 1; Vvec = w.Z; ldv = k; v_scaled = 0; }
@@ -536,6 +545,66 @@ extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_c
     return TN_ERR_NOCONV;
   }
   return TN_OK;
+}
+
+// ---- scratch of the allocation-free SVD entry point (tn_svd_noalloc) ----
+// One array per GPU, held by the loaded code object until the process ends.
+// In host code `scratch` names a host-side stand-in, not the device array:
+// its device address is fetched per device with hipGetSymbolAddress and
+// only that address is handed to kernels.
+constexpr size_t kSvdScratchBytes = (size_t)TN_SVD_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kSvdScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxScratchDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current) and kept.
+static int scratch_address(void** out) {
+  int dev = -1;
+  TN_HIP(hipGetDevice(&dev));
+  TN_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "tn_svd_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error(TN_ERR_HIP, "tn_svd_noalloc: hipGetSymbolAddress failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return TN_ERR_HIP;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return TN_OK;
+}
+
+}  // namespace tn
+
+extern "C" int tn_svd(const void* A, const tn_modes* a_rows, const tn_modes* a_cols, int64_t m,
+                      int64_t n, void* U, double* S, void* Vt, int dtype, double tol,
+                      int max_sweeps, int* sweeps_out, void* ws, size_t ws_bytes, void* stream) {
+  return svd_impl(A, a_rows, a_cols, m, n, U, S, Vt, dtype, tol, max_sweeps, sweeps_out, ws,
+                  ws_bytes, stream, true);
+}
+
+extern "C" int tn_svd_noalloc(const void* A, const tn_modes* a_rows, const tn_modes* a_cols,
+                              int64_t m, int64_t n, void* U, double* S, void* Vt, int dtype,
+                              double tol, int max_sweeps, int* sweeps_out, void* stream) {
+  TN_REQUIRE(m >= 0 && n >= 0, "tn_svd_noalloc: negative size");
+  const size_t need = tn_svd_workspace(m, n, dtype);
+  if (need > kSvdScratchBytes) {
+    set_error(TN_ERR_WORKSPACE,
+              "tn_svd_noalloc: a %lld x %lld SVD needs %zu bytes of scratch, more than the "
+              "%zu bytes built in",
+              (long long)m, (long long)n, need, kSvdScratchBytes);
+    return TN_ERR_WORKSPACE;
+  }
+  void* ws = nullptr;
+  TN_TRY(scratch_address(&ws));
+  return svd_impl(A, a_rows, a_cols, m, n, U, S, Vt, dtype, tol, max_sweeps, sweeps_out, ws,
+                  kSvdScratchBytes, stream, false);
 }
 
 // ---------------------------------------------------------------------------
