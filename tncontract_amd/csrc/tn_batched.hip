@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(256) bjac_eig(const double* __restrict__ Gpart
     }
   }
   __syncthreads();
-  const double off = eig_off(sm, J.neg2);
+  const double off = eig_off(sm, J.tol, J.neg2);
   if (tid == 0) atomicMax(offmax + j, dbits(off));
   if (off <= J.tol) {
     if (tid == 0) skip[item] = 1;
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
   for (int e = 0; e < 4; ++e) sm.G[0][16 * ti + (lane >> 4) + 4 * e][16 * tj + (lane & 15)] = acc[e];
   for (int e = tid; e < BJ2 * BJ2; e += 256) sm.Q[0][e / BJ2][e % BJ2] = (e / BJ2 == e % BJ2) ? 1.0 : 0.0;
   __syncthreads();
-  const double off = eig_off(sm, J.neg2);
+  const double off = eig_off(sm, J.tol, J.neg2);
   if (tid == 0) atomicMax(offmax + j, dbits(off));
   if (off <= J.tol) return;
   if (work && tid == 0) {

@@ -151,7 +151,6 @@ __global__ void __launch_bounds__(256) jac_eig_apply(double* __restrict__ Wv, in
                                                      const double* __restrict__ Gpart, double tol,
                                                      unsigned long long* __restrict__ offmax) {
   __shared__ EigLds sm;
-  __shared__ ApplyTile Tw;
   const int pi = blockIdx.x, ch = blockIdx.y;
   int I, J;
   pair_blocks(pi, s, nblk, I, J);
@@ -163,13 +162,13 @@ __global__ void __launch_bounds__(256) jac_eig_apply(double* __restrict__ Wv, in
     base = Z; ld = nv; rs = (int64_t)(ch - nchW) * JCH; re = min(nv, rs + JCH);
   }
   eig_load(sm, Gpart + (int64_t)pi * nchG * (J2 * J2), nchG);
-  double x[J2];
-  apply_tile_fetch(base, ld, nv, rs, re, I, J, x);   // in flight during the rounds
-  const double off = eig_off(sm);
+  double x[8][4];
+  apply_rows_fetch(base, ld, nv, rs, re, I, J, x);   // in flight during the rounds
+  const double off = eig_off(sm, tol);
   if (ch == 0 && threadIdx.x == 0) atomicMax(offmax, dbits(off));
   if (off <= tol) return;
   const int cur = eig_rotate(sm, tol);
-  apply_tile_finish(base, ld, nv, rs, re, I, J, x, sm.Q[cur], Tw);
+  apply_rows_finish(base, ld, nv, rs, re, I, J, x, sm.Q[cur]);
 }
 
 // sigma_j = ||Wv_j|| (one block per vector)
