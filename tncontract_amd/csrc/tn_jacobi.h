@@ -39,7 +39,7 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 struct EigLds {
-  double G[2][J2][J2 + 1];
+  double G[2][J2][J2 + 4];   // row padding: fewer LDS bank conflicts in the rounds (measured)
   double Q[2][J2][J2 + 1];
   double red[4];
 };
@@ -316,46 +316,66 @@ __device__ inline void apply_tile_finish(double* __restrict__ base, int64_t ld, 
   }
 }
 
-// Rows [rs, re) (re - rs <= 256) of the pair's 32 vectors (vector v at
+// Rows [rs, re) (re - rs <= 64 NT) of the pair's 32 vectors (vector v at
 // base + v * ld, nv vectors) <- rows . Q, as the transposed product
 // (Q^T X^T) on v_mfma_f64_16x16x4: the B operand of each 16 x 16 output tile
 // is 16 consecutive rows of 4 vectors, read straight from memory (four
 // 128-byte runs per load), the A operand is Q^T from LDS, and the output
 // tile is 16 consecutive rows of 4 vectors again, stored straight back: no
-// LDS staging and no barrier.  Wave w covers rows rs + 64 w + [0, 64) as 4
-// row tiles; lane (lr = lane % 16, lk = lane / 16).
-//   apply_rows_fetch:  x[ks][nt] = row rs + 64 w + 16 nt + lr of vector
+// LDS staging and no barrier.  Wave w covers rows rs + 16 NT w + [0, 16 NT)
+// as NT row tiles; lane (lr = lane % 16, lk = lane / 16).
+//   apply_rows_fetch:  x[ks][nt] = row rs + 16 NT w + 16 nt + lr of vector
 //                      4 ks + lk of the pair (issued early: in flight while
 //                      the caller computes Q);
-//   apply_rows_finish: 64 MFMAs per wave (2 vector tiles x 4 row tiles x 8
-//                      k-steps) and the stores.
+//   apply_rows_finish: 16 NT MFMAs per wave (2 vector tiles x NT row tiles x
+//                      8 k-steps) and the stores.
+// Whether the chunk is a whole 64 NT rows and all 32 vectors of the pair
+// exist (block-uniform): then the loads and stores need no bounds test.
+template <int NT>
+__device__ __forceinline__ bool apply_rows_full(int64_t nv, int64_t rs, int64_t re, int I, int J) {
+  return re - rs == 64 * NT && (int64_t)max(I, J) * JB + JB <= nv;
+}
+
+template <int NT>
 __device__ __forceinline__ void apply_rows_fetch(const double* __restrict__ base, int64_t ld,
                                                  int64_t nv, int64_t rs, int64_t re, int I,
-                                                 int J, double (&x)[8][4]) {
+                                                 int J, double (&x)[8][NT]) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
+  const int64_t r0 = rs + 16 * NT * wave + lr;
+  if (apply_rows_full<NT>(nv, rs, re, I, J)) {
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const double* p = base + pair_vec(4 * ks + lk, I, J) * ld + r0;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) x[ks][nt] = p[16 * nt];
+    }
+    return;
+  }
 #pragma unroll
   for (int ks = 0; ks < 8; ++ks) {
     const int64_t v = pair_vec(4 * ks + lk, I, J);
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      const int64_t r = rs + 64 * wave + 16 * nt + lr;
+    for (int nt = 0; nt < NT; ++nt) {
+      const int64_t r = r0 + 16 * nt;
       x[ks][nt] = (v < nv && r < re) ? base[v * ld + r] : 0.0;
     }
   }
 }
 
+template <int NT>
 __device__ __forceinline__ void apply_rows_finish(double* __restrict__ base, int64_t ld,
                                                   int64_t nv, int64_t rs, int64_t re, int I,
-                                                  int J, const double (&x)[8][4],
+                                                  int J, const double (&x)[8][NT],
                                                   const double (*Q)[J2 + 1]) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
-  f64x4 acc[2][4];
+  const int64_t r0 = rs + 16 * NT * wave + lr;
+  f64x4 acc[2][NT];
 #pragma unroll
   for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f64x4{0, 0, 0, 0};
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f64x4{0, 0, 0, 0};
 #pragma unroll
   for (int ks = 0; ks < 8; ++ks) {
 #pragma unroll
@@ -363,19 +383,30 @@ __device__ __forceinline__ void apply_rows_finish(double* __restrict__ base, int
       // A[i][k] = Q^T[16 mt + i][4 ks + k] = Q[4 ks + k][16 mt + i]
       const double a = Q[4 * ks + lk][16 * mt + lr];
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt)
+      for (int nt = 0; nt < NT; ++nt)
         acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x[ks][nt], acc[mt][nt], 0, 0, 0);
     }
   }
   // output tile (mt, nt): lane holds vector 16 mt + lk + 4 e, row 16 nt + lr
+  if (apply_rows_full<NT>(nv, rs, re, I, J)) {
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double* p = base + pair_vec(16 * mt + lk + 4 * e, I, J) * ld + r0;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) p[16 * nt] = acc[mt][nt][e];
+      }
+    return;
+  }
 #pragma unroll
   for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int64_t v = pair_vec(16 * mt + lk + 4 * e, I, J);
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        const int64_t r = rs + 64 * wave + 16 * nt + lr;
+      for (int nt = 0; nt < NT; ++nt) {
+        const int64_t r = r0 + 16 * nt;
         if (v < nv && r < re) base[v * ld + r] = acc[mt][nt][e];
       }
     }

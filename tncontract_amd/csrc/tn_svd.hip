@@ -32,9 +32,9 @@
 
 namespace tn {
 
-constexpr int JCH = 256;     // rows per chunk of jac_eig_apply
+constexpr int JNT = 2;                // row tiles per wave in jac_eig_apply's apply
+constexpr int JCH = 64 * JNT;         // rows per chunk of jac_eig_apply
 constexpr int GCH = 128;     // rows per Gram partial of jac_gram (twice the blocks)
-static_assert(JCH == 256, "apply_tile_load/finish cover 256 rows per block");
 
 // Partial Gram P^T P of the pair's 32 vectors over rows [r0, r1) into out
 // (32 x 32, row stride ostride; global or LDS), v_mfma_f64_16x16x4 from a
@@ -162,13 +162,13 @@ __global__ void __launch_bounds__(256) jac_eig_apply(double* __restrict__ Wv, in
     base = Z; ld = nv; rs = (int64_t)(ch - nchW) * JCH; re = min(nv, rs + JCH);
   }
   eig_load(sm, Gpart + (int64_t)pi * nchG * (J2 * J2), nchG);
-  double x[8][4];
-  apply_rows_fetch(base, ld, nv, rs, re, I, J, x);   // in flight during the rounds
+  double x[8][JNT];
+  apply_rows_fetch<JNT>(base, ld, nv, rs, re, I, J, x);   // in flight during the rounds
   const double off = eig_off(sm, tol);
   if (ch == 0 && threadIdx.x == 0) atomicMax(offmax, dbits(off));
   if (off <= tol) return;
   const int cur = eig_rotate(sm, tol);
-  apply_rows_finish(base, ld, nv, rs, re, I, J, x, sm.Q[cur]);
+  apply_rows_finish<JNT>(base, ld, nv, rs, re, I, J, x, sm.Q[cur]);
 }
 
 // sigma_j = ||Wv_j|| (one block per vector)
@@ -191,18 +191,36 @@ __global__ void __launch_bounds__(256) vec_norms(const double* __restrict__ Wv, 
 // NaN ranks as -inf so that the ranks stay a permutation of 0..k-1.
 __device__ __forceinline__ double rank_key(double x) { return x == x ? x : -INFINITY; }
 
-__global__ void __launch_bounds__(1024) rank_sort(const double* __restrict__ sig, int64_t k,
-                                                  int* __restrict__ perm, double* __restrict__ S) {
-  for (int64_t j = threadIdx.x; j < k; j += 1024) {
-    const double sj = rank_key(sig[j]);
-    int64_t rank = 0;
-    for (int64_t i = 0; i < k; ++i) {
-      const double si = rank_key(sig[i]);
-      rank += (si > sj || (si == sj && i < j)) ? 1 : 0;
+// One thread per j (grid ceil(k / 256) x 256); the keys are compared from
+// LDS tiles of 2048 (every thread reads the same key: a broadcast).
+constexpr int RANK_TILE = 2048;
+__global__ void __launch_bounds__(256) rank_sort(const double* __restrict__ sig, int64_t k,
+                                                 int* __restrict__ perm, double* __restrict__ S) {
+  __shared__ double tile[RANK_TILE];
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const double sj = j < k ? rank_key(sig[j]) : 0.0;
+  int64_t rank = 0;
+  for (int64_t t0 = 0; t0 < k; t0 += RANK_TILE) {
+    const int tn = (int)min<int64_t>(RANK_TILE, k - t0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < tn; i += 256) tile[i] = rank_key(sig[t0 + i]);
+    __syncthreads();
+    int cnt = 0;
+    for (int i = 0; i < tn; ++i) {
+      const double si = tile[i];
+      cnt += (si > sj || (si == sj && t0 + i < j)) ? 1 : 0;
     }
+    rank += cnt;
+  }
+  if (j < k) {
     perm[rank] = (int)j;
     S[rank] = sig[j];
   }
+}
+
+static void launch_rank_sort(const double* sig, int64_t k, int* perm, double* S, hipStream_t st) {
+  hipLaunchKernelGGL(rank_sort, dim3((unsigned)ceil_div(std::max<int64_t>(k, 1), 256)), dim3(256),
+                     0, st, sig, k, perm, S);
 }
 
 // U (m x k row-major) and Vt (k x n row-major).  Column j of U is vector
@@ -478,7 +496,7 @@ static int svd_impl(const void* A, const tn_modes* a_rows, const tn_modes* a_col
     // X' = Q R; one-sided Jacobi on the rows of R (R^T Z = W''), then
     // X' = (Q Z) Sigma (W''/Sigma)^T and X = X' P^T.
     hipLaunchKernelGGL(vec_norms, dim3((unsigned)k), dim3(256), 0, st, w.Wv, L, w.sig);
-    hipLaunchKernelGGL(rank_sort, dim3(1), dim3(1024), 0, st, w.sig, k, w.vp, w.S);
+    launch_rank_sort(w.sig, k, w.vp, w.S, st);
     hipLaunchKernelGGL(permute_vectors, dim3((unsigned)ceil_div(L * k, 256)), dim3(256), 0, st,
                        w.Wv, L, k, w.vp, w.QZ);
     TN_CHECK_LAUNCH();
@@ -527,7 +545,7 @@ static int svd_impl(const void* A, const tn_modes* a_rows, const tn_modes* a_col
     else { Uvec = w.Z; ldu = k; u_scaled = 0; Vvec = w.Wv; ldv = L; v_scaled = 1; }
   }
   hipLaunchKernelGGL(vec_norms, dim3((unsigned)k), dim3(256), 0, st, orth_src, orth_len, w.sig);
-  hipLaunchKernelGGL(rank_sort, dim3(1), dim3(1024), 0, st, w.sig, k, w.perm, S);
+  launch_rank_sort(w.sig, k, w.perm, S, st);
   TN_CHECK_LAUNCH();
   unsigned ob = (unsigned)ceil_div(m * k + k * n, 256);
   if (dtype == TN_F64)
@@ -744,7 +762,7 @@ extern "C" int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes
   // indefinite at ~eps_32 |K|)
   const double eps_in = dtype == TN_F64 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
   hipLaunchKernelGGL(psd_diag, dim3(1), dim3(256), 0, st, Wv, n, eps_in, e.diag, e.delta);
-  hipLaunchKernelGGL(rank_sort, dim3(1), dim3(1024), 0, st, e.diag, n, e.p, e.Sd);
+  launch_rank_sort(e.diag, n, e.p, e.Sd, st);
   const unsigned nb = (unsigned)ceil_div(n * n, 256);
   hipLaunchKernelGGL(psd_permute, dim3(nb), dim3(256), 0, st, Wv, n, e.p, e.delta, e.B);
   TN_CHECK_LAUNCH();
@@ -771,7 +789,7 @@ extern "C" int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes
   bool converged = false;
   TN_TRY(jacobi_run(Wv, n, nullptr, n, e.jw, tol, max_sweeps, sweeps_out, &converged, st));
   hipLaunchKernelGGL(vec_norms, dim3((unsigned)n), dim3(256), 0, st, Wv, n, e.jw.sig);
-  hipLaunchKernelGGL(rank_sort, dim3(1), dim3(1024), 0, st, e.jw.sig, n, e.jw.perm, e.jw.S);
+  launch_rank_sort(e.jw.sig, n, e.jw.perm, e.jw.S, st);
   hipLaunchKernelGGL(eig_out_kernel, dim3(nb), dim3(256), 0, st, Wv, n, e.p, e.jw.perm, e.jw.S,
                      e.delta, U, lam);
   TN_CHECK_LAUNCH();
