@@ -156,13 +156,14 @@ __global__ void __launch_bounds__(256) gemm_grouped(const GJob* __restrict__ job
 
 // ---------------------------------------------------------------- batched Jacobi
 constexpr int BJB = 16, BJ2 = 32, BCH = 256;
+constexpr int BANT = 2, BACH = 64 * BANT;   // row tiles per wave / rows per chunk of bjac_apply
 static_assert(BJB == JB && BJ2 == J2, "batched pairs use the shared pair layout");
-static_assert(BCH == 256, "apply_mfma covers 256 rows per block");
 
 struct BJob {
   int64_t wv;     // offset of the job's vectors (k vectors of length L)
   int64_t z;      // offset of the job's rotation accumulator (k x k)
-  int32_t L, k, nblk, nch, nchz;
+  int32_t L, k, nblk, nch;
+  int32_t ncha, nchaz;   // row chunks of W and Z in bjac_apply (BACH rows)
   int32_t gram_base, eig_base;
   double tol;
   // squared norm below which a vector is negligible (pairs of two such
@@ -320,7 +321,6 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
                                                   const int* __restrict__ skip,
                                                   unsigned long long* __restrict__ work) {
   __shared__ double Q[BJ2][BJ2 + 1];
-  __shared__ ApplyTile Tw;
   const int item = blockIdx.x;
   const int j = a_job[item];
   if (done[j]) return;
@@ -331,17 +331,26 @@ __global__ void __launch_bounds__(256) bjac_apply(double* __restrict__ Wv, doubl
   const int s = (*gbase + goff) % (J.nblk - 1);
   const int I = brr(pi, s, J.nblk), K2 = brr(J.nblk - 1 - pi, s, J.nblk);
   const double* Qg = Qall + (int64_t)eidx * BJ2 * BJ2;
-  if (work && threadIdx.x == 0) {
-    const int64_t r = (int64_t)(ch < J.nch ? ch : ch - J.nch) * BCH;
-    atomicAdd(work + 2, (unsigned long long)(min((int64_t)(ch < J.nch ? J.L : J.k), r + BCH) - r));
+  const bool onW = ch < J.ncha;
+  double* base = onW ? Wv + J.wv : Zb + J.z;
+  const int64_t ld = onW ? J.L : J.k;
+  const int64_t r = (int64_t)(onW ? ch : ch - J.ncha) * BACH;
+  const int64_t re = min(ld, r + BACH);
+  if (work && threadIdx.x == 0) atomicAdd(work + 2, (unsigned long long)(re - r));
+  // Q's loads and the chunk's loads are all issued before Q goes into LDS
+  constexpr int PER = BJ2 * BJ2 / 256;
+  double qv[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) qv[i] = Qg[threadIdx.x + 256 * i];
+  double x[8][BANT];
+  apply_rows_fetch<BANT>(base, ld, J.k, r, re, I, K2, x);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    Q[e / BJ2][e % BJ2] = qv[i];
   }
-  if (ch < J.nch) {
-    const int64_t r = (int64_t)ch * BCH;
-    apply_mfma(Wv + J.wv, J.L, J.k, r, min((int64_t)J.L, r + BCH), I, K2, Qg, Q, Tw);
-  } else {
-    const int64_t r = (int64_t)(ch - J.nch) * BCH;
-    apply_mfma(Zb + J.z, J.k, J.k, r, min((int64_t)J.k, r + BCH), I, K2, Qg, Q, Tw);
-  }
+  lds_barrier();
+  apply_rows_finish<BANT>(base, ld, J.k, r, re, I, K2, x, Q);
 }
 
 // One Jacobi step for batches whose jobs fit one row chunk (L <= BCH and
@@ -362,7 +371,6 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
                                                   unsigned long long* __restrict__ work) {
   __shared__ double Ps[64][BJ2 + 1];
   __shared__ EigLds sm;
-  __shared__ ApplyTile Tw;
   const int item = blockIdx.x, tid = threadIdx.x;
   const int j = e_job[item];
   if (done[j]) return;
@@ -374,9 +382,9 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
   const int ti = wave >> 1, tj = wave & 1;
   double* W = Wv + J.wv;
   double* Z = Zb + J.z;
-  double x[BJ2], z[BJ2];
-  apply_tile_fetch(W, J.L, J.k, 0, J.L, I, K2, x);  // in flight during the Gram and rounds
-  apply_tile_fetch(Z, J.k, J.k, 0, J.k, I, K2, z);
+  double x[8][4], z[8][4];
+  apply_rows_fetch<4>(W, J.L, J.k, 0, J.L, I, K2, x);  // in flight during the Gram and rounds
+  apply_rows_fetch<4>(Z, J.k, J.k, 0, J.k, I, K2, z);
   f64x4b acc = {0, 0, 0, 0};
   bgram_rows(W, J.L, J.k, I, K2, 0, J.L, Ps, acc);
 #pragma unroll
@@ -392,8 +400,8 @@ __global__ void __launch_bounds__(256) bjac_fused(double* __restrict__ Wv, doubl
     atomicAdd(work + 2, (unsigned long long)(J.L + J.k));
   }
   const int cur = eig_rotate(sm, J.tol, J.neg2);
-  apply_tile_finish(W, J.L, J.k, 0, J.L, I, K2, x, sm.Q[cur], Tw);
-  apply_tile_finish(Z, J.k, J.k, 0, J.k, I, K2, z, sm.Q[cur], Tw);
+  apply_rows_finish<4>(W, J.L, J.k, 0, J.L, I, K2, x, sm.Q[cur]);
+  apply_rows_finish<4>(Z, J.k, J.k, 0, J.k, I, K2, z, sm.Q[cur]);
 }
 
 // After global step g: retire jobs whose sweep just ended with off <= tol.
@@ -638,7 +646,7 @@ static size_t batched_ws_bytes(const std::vector<int64_t>& ms, const std::vector
     z += (size_t)(k * k);
     gi += (size_t)np * ceil_div(L, BCH);
     ei += (size_t)np;
-    ai += (size_t)np * (ceil_div(L, BCH) + ceil_div(k, BCH));
+    ai += (size_t)np * (ceil_div(L, BACH) + ceil_div(k, BACH));
     sv += (size_t)k;
   }
   Arena a{nullptr, 0, 0};
@@ -718,7 +726,8 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
     b.nblk = (int)ceil_div(k, BJB);
     b.nblk += b.nblk % 2;
     b.nch = (int)ceil_div(L, BCH);
-    b.nchz = (int)ceil_div(k, BCH);
+    b.ncha = (int)ceil_div(L, BACH);
+    b.nchaz = (int)ceil_div(k, BACH);
     b.gram_base = (int)g_job.size();
     b.eig_base = (int)e_job.size();
     // Orthogonality to L * eps (LAPACK dgesvj's threshold) for the SVD jobs;
@@ -734,7 +743,7 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
       for (int c = 0; c < b.nch; ++c) { g_job.push_back(j); g_pair.push_back(p); g_chunk.push_back(c); }
       e_job.push_back(j);
       e_pair.push_back(p);
-      for (int c = 0; c < b.nch + b.nchz; ++c) { a_job.push_back(j); a_pair.push_back(p); a_chunk.push_back(c); }
+      for (int c = 0; c < b.ncha + b.nchaz; ++c) { a_job.push_back(j); a_pair.push_back(p); a_chunk.push_back(c); }
     }
     bj[j] = b;
     sbase[j] = sv;

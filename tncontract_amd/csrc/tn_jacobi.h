@@ -245,77 +245,6 @@ __device__ inline int eig_rotate(EigLds& sm, double tol, double neg2 = 0.0) {
   return 0;
 }
 
-// Per-wave LDS staging tiles of apply_mfma (64 rows x 32 vectors per wave).
-typedef double ApplyTile[4][64][J2 + 1];
-
-// Rows [rs, re) (re - rs <= 256) of the pair's 32 vectors (vector v at
-// base + v * ld, nv vectors) <- rows . Q, in two halves so that a caller can
-// put other work between them while the loads are in flight:
-//   apply_tile_fetch:  thread t (wave w, lane l) loads row rs + 64 w + l of
-//                      the 32 vectors into registers (64 consecutive rows per
-//                      load across a wave);
-//   apply_tile_finish: stores the rows into the wave's LDS tile, multiplies
-//                      the tiles by Q (32 x 32 in LDS, row stride J2 + 1)
-//                      with v_mfma_f64_16x16x4 (A operands from the tile, B
-//                      from Q), writes the product back into the tile and
-//                      stores it column by column.
-// 256 threads; LDS-only barriers inside apply_tile_finish.
-__device__ __forceinline__ void apply_tile_fetch(const double* __restrict__ base, int64_t ld,
-                                                 int64_t nv, int64_t rs, int64_t re, int I,
-                                                 int J, double (&x)[J2]) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t r = rs + 64 * wave + lane;
-#pragma unroll
-  for (int c = 0; c < J2; ++c) {
-    const int64_t v = pair_vec(c, I, J);
-    x[c] = (v < nv && r < re) ? base[v * ld + r] : 0.0;
-  }
-}
-
-__device__ inline void apply_tile_finish(double* __restrict__ base, int64_t ld, int64_t nv,
-                                         int64_t rs, int64_t re, int I, int J,
-                                         const double (&x)[J2], const double (*Q)[J2 + 1],
-                                         ApplyTile& Tw) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int lr = lane & 15, lk = lane >> 4;
-  auto& T = Tw[wave];
-  const int64_t r = rs + 64 * wave + lane;
-#pragma unroll
-  for (int c = 0; c < J2; ++c) T[lane][c] = x[c];
-  lds_barrier();
-  double a[4][8];
-#pragma unroll
-  for (int ks = 0; ks < 8; ++ks)
-#pragma unroll
-    for (int t = 0; t < 4; ++t) a[t][ks] = T[16 * t + lr][4 * ks + lk];
-  f64x4 acc[4][2];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = f64x4{0, 0, 0, 0};
-#pragma unroll
-  for (int ks = 0; ks < 8; ++ks) {
-#pragma unroll
-    for (int ct = 0; ct < 2; ++ct) {
-      const double b = Q[4 * ks + lk][16 * ct + lr];
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-        acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][ks], b, acc[t][ct], 0, 0, 0);
-    }
-  }
-  lds_barrier();
-#pragma unroll
-  for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) T[16 * t + lk + 4 * e][16 * ct + lr] = acc[t][ct][e];
-  lds_barrier();
-#pragma unroll
-  for (int c = 0; c < J2; ++c) {
-    const int64_t v = pair_vec(c, I, J);
-    if (v < nv && r < re) base[v * ld + r] = T[lane][c];
-  }
-}
-
 // Rows [rs, re) (re - rs <= 64 NT) of the pair's 32 vectors (vector v at
 // base + v * ld, nv vectors) <- rows . Q, as the transposed product
 // (Q^T X^T) on v_mfma_f64_16x16x4: the B operand of each 16 x 16 output tile
@@ -410,27 +339,6 @@ __device__ __forceinline__ void apply_rows_finish(double* __restrict__ base, int
         if (v < nv && r < re) base[v * ld + r] = acc[mt][nt][e];
       }
     }
-}
-
-// apply_tile_fetch + (Q from the row-major Qg in global memory into the LDS
-// buffer Q) + apply_tile_finish.  Q's loads are issued first and all of
-// them before any LDS store, so storing Q waits only for Q (a load-store
-// loop would wait for the whole chunk fetch and then once per Q element).
-__device__ inline void apply_mfma(double* __restrict__ base, int64_t ld, int64_t nv, int64_t rs,
-                                  int64_t re, int I, int J, const double* __restrict__ Qg,
-                                  double (*Q)[J2 + 1], ApplyTile& Tw) {
-  constexpr int PER = J2 * J2 / 256;
-  double qv[PER];
-#pragma unroll
-  for (int i = 0; i < PER; ++i) qv[i] = Qg[threadIdx.x + 256 * i];
-  double x[J2];
-  apply_tile_fetch(base, ld, nv, rs, re, I, J, x);
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int e = threadIdx.x + 256 * i;
-    Q[e / J2][e % J2] = qv[i];
-  }
-  apply_tile_finish(base, ld, nv, rs, re, I, J, x, Q, Tw);
 }
 
 }  // namespace tn
