@@ -43,7 +43,7 @@ NOTES = {"jac_eig_apply": "pair eigensolve (FP64 rotations, 31 dependent rounds 
 # HBM bytes per launch (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, calibrated on a
 # streaming kernel) measured for the chi=512 sweep's n=1024 Jacobi launches;
 # profiles/jac_eig_chi512_pmc_traffic.md.  Used when --traffic is not given.
-MEASURED_TRAFFIC = {"jac_eig_apply": 17049600}
+MEASURED_TRAFFIC = {"jac_eig_apply": 16738191, "jac_gram": 10560512}
 
 
 def parse():
@@ -292,7 +292,11 @@ def roofline_from_prof(prof, traffic=None, measured=None):
     measured = measured or {}
     dom = max(prof, key=lambda t: prof[t][0])
     ms, n, fl, by = prof[dom]
-    if dom in PEAK_TFLOPS and fl > 0:
+    # the binding roof by arithmetic intensity: below the ridge point
+    # (peak FLOP/s / 8 TB/s) HBM bounds the kernel, above it MFMA does
+    mfma_bound = dom in PEAK_TFLOPS and fl > 0 and (
+        by <= 0 or fl / by >= PEAK_TFLOPS[dom] * 1e12 / 8.0e12)
+    if mfma_bound:
         ach = fl / (ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_TFLOPS[dom],
                 "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dom],
@@ -306,6 +310,9 @@ def roofline_from_prof(prof, traffic=None, measured=None):
                 "traffic": traffic if traffic is not None else measured.get(dom),
                 "avg_launch_us": ms * 1e3 / max(n, 1), "launches": n,
                 "bytes_per_launch": by / max(n, 1)}
+        if fl > 0:
+            roof["flops_per_launch"] = fl / max(n, 1)
+            roof["flop_per_byte"] = fl / by
     if dom in NOTES:
         roof["note"] = NOTES[dom]
     return roof

@@ -375,10 +375,11 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
         // a kernel boundary between the partials and the pair eigensolve is
         // cheaper than a device-scope fence + arrival counter in the Gram
         // kernel (measured).  Work: the inner sweep (31 rounds x 9216 flops)
-        // once per pair plus the apply; bytes: partials per block + chunks
+        // once per pair plus the apply; HBM bytes: each pair's partials once
+        // (the other row blocks of the pair re-read them from L2) + the
+        // chunks read and written
         ProfScope p(PROF_JAC_EIG, st, 31.0 * 9216.0 * npairs + apply_fl,
-                    8.0 * (double)nchG * J2 * J2 * npairs * (nchW + nchZ) +
-                        16.0 * (double)(L + kz) * J2 * npairs);
+                    8.0 * (double)nchG * J2 * J2 * npairs + 16.0 * (double)(L + kz) * J2 * npairs);
         hipLaunchKernelGGL(jac_eig_apply, dim3(npairs, nchW + nchZ), dim3(256), 0, st, Wv, L, Z,
                            k, s, nblk, nchW, nchG, w.Gpart, tol, w.off);
       }

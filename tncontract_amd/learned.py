@@ -141,7 +141,15 @@ def randn(shape, dtype=torch.float64, seed=0):
 
 
 def _orth(Y):
-    q, _ = tsr._qr_device(L.convert(Y, torch.float64), 1)
+    """Orthonormal basis of the columns of the tall device matrix Y: two-pass
+    Cholesky QR (GEMMs), or Householder QR where Y is too ill-conditioned
+    for it.  A basis that misses directions of Y shows up in the sketch's
+    a-posteriori residual bound, never in an accepted result."""
+    Y64 = L.convert(Y, torch.float64)
+    res = tsr._cholqr_device(Y64, 1)
+    if res is not None:
+        return res[0]
+    q, _ = tsr._qr_device(Y64, 1)
     return q
 
 
@@ -149,7 +157,7 @@ def rsvd(A, rank, oversample=10, power_iters=2, seed=0):
     """Randomized thin SVD of the m x n device matrix A (FP64 arithmetic; FP32
     data is exact in FP64) with p = rank + oversample sketch columns and
     `power_iters` power iterations, each application of A or A^T followed by a
-    Householder re-orthogonalisation.  Returns (U m x p, s p, Vt p x n)."""
+    re-orthonormalisation (_orth).  Returns (U m x p, s p, Vt p x n)."""
     m, n = A.shape
     p = min(rank + oversample, m, n)
     A64 = L.convert(A, torch.float64)
@@ -165,40 +173,37 @@ def rsvd(A, rank, oversample=10, power_iters=2, seed=0):
     return U, s, vt
 
 
-def residual_norm_estimate(A, U, s, Vt, block=8, iters=4, seed=1):
-    """Lower estimate of ||A - U diag(s) Vt||_2 by block power iteration
-    (Gaussian start of `block` columns, `iters` rounds of R^T R with a
-    Householder re-orthogonalisation): the largest singular value of R X for
-    the final orthonormal X.  Returns (estimate, ||R||_F)."""
+# posterior bound on the sketch's residual (Halko, Martinsson and Tropp,
+# SIAM Rev. 53 (2011), Lemma 4.1): for Gaussian w_1..w_r,
+# ||(I - Q Q^T) A||_2 <= 10 sqrt(2/pi) max_i ||(I - Q Q^T) A w_i|| with
+# probability at least 1 - 10^-r
+POSTERIOR_TESTS = 10
+POSTERIOR_FACTOR = 10.0 * np.sqrt(2.0 / np.pi)
+
+
+def residual_bound(A, U, ntest=POSTERIOR_TESTS, seed=1):
+    """Upper bound, with probability at least 1 - 10^-ntest, on the spectral
+    norm of the sketch's residual A - U U^T A (U: the sketch's orthonormal
+    m x p basis; A - U diag(s) Vt equals it, s and Vt being the full SVD of
+    U^T A), from ntest Gaussian test vectors: three thin products, no
+    factorisation."""
     con = tsr._contract_data
     A64 = L.convert(A, torch.float64)
-    rec = con(con(U, [1], L.diag_embed(s), [0]), [1], Vt, [0])
-    R = L.axpby(A64, rec, 1.0, -1.0)
-    frob = float(np.sqrt(L.sumsq(R)))
-    if frob == 0.0:
-        return 0.0, 0.0
-    X = _orth(randn((R.shape[1], min(block, R.shape[1])), dtype=torch.float64, seed=seed))
-    for _ in range(iters):
-        X = _orth(con(R, [0], con(R, [1], X, [0]), [0]))
-    _, sx, _ = tsr._svd_device(con(R, [1], X, [0]), 1)
-    return float(sx[0].item()), frob
+    W = randn((A64.shape[1], ntest), dtype=torch.float64, seed=seed)
+    Y = con(A64, [1], W, [0])                                   # m x ntest
+    E = L.axpby(Y, con(U, [1], con(U, [0], Y, [0]), [0]), 1.0, -1.0)
+    col2 = torch.diagonal(con(E, [0], E, [0])).cpu().numpy()   # ||E_i||^2
+    return POSTERIOR_FACTOR * float(np.sqrt(max(float(np.max(col2)), 0.0)))
 
 
-# safety factor on the power estimate of the residual's spectral norm (the
-# estimate is a lower bound; four rounds of an 8-column block bring it within
-# a few per cent of the norm for the residuals of these sketches)
-RESIDUAL_SAFETY = 1.5
-
-
-def certified_svd(A, rank, threshold, oversample=10, power_iters=2, seed=0,
-                  safety=RESIDUAL_SAFETY, full_fraction=0.5):
+def certified_svd(A, rank, threshold, oversample=10, power_iters=2, seed=0, full_fraction=0.5):
     """Truncation SVD of the m x n device matrix A for the keep rule
     sigma / sigma_0 > threshold, by the randomized SVD at the predicted
     `rank` when it certifies that rule, else by the full Jacobi SVD.
     Returns (U, s, Vt, info): U m x p, s p (descending, FP64), Vt p x n with
     p = the sketch size or min(m, n); every singular value of A above
     threshold * sigma_0 is among s.  info: {"path": "sketch" | "fallback" |
-    "full", "rank": rank, "residual": estimate * safety / sigma_0 or None}.
+    "full", "rank": rank, "residual": the residual bound / sigma_0 or None}.
     "full": the sketch would hold more than full_fraction * min(m, n) columns,
     so the full SVD runs without trying it."""
     m, n = A.shape
@@ -209,8 +214,7 @@ def certified_svd(A, rank, threshold, oversample=10, power_iters=2, seed=0,
         sh = s.cpu().numpy()
         s0 = sh[0] if sh.size else 0.0
         if s0 > 0.0 and np.all(np.isfinite(sh)):
-            est, _ = residual_norm_estimate(A, U, s, vt, seed=seed + 1)
-            bound = safety * est
+            bound = residual_bound(A, U, seed=seed + 1)
             info["residual"] = bound / s0
             cut = threshold * s0
             certified = (bound <= cut and int(np.count_nonzero(sh > cut)) < sh.size
