@@ -17,9 +17,11 @@ MPO application + compression of the same resident input state.
             the test suite.
   roofline -- the dominant kernel of the fast sweep, timed live with HIP
             events on its launch stream (library launch timers).
-  cpu_baseline -- the NumPy FP64 oracle on a bounded sample (one bulk site of
-            the reference algorithm at the workload's shapes), scaled to a
+  cpu_baseline -- the NumPy FP64 oracle's full reference sweep of a 22-site
+            chain of the workload (a bounded sample), scaled to the 64-site
             sweep by the per-site flop model.
+  oracle_check -- the fast and plain GPU paths on that 22-site chain
+            against the oracle's result (state error, bond dimensions).
 
 Prints ONE JSON line.
 """
@@ -121,13 +123,31 @@ def flops_per_site_plain(chi_l, chi_r, D_l, D_r, d, chi_out_r, qb_l):
     return qr + gemm + svd
 
 
-def cpu_baseline(N, d, chi, D=5, window=4):
-    """Time the oracle on `window` consecutive bulk sites of the workload's
-    sweep (the reference algorithm's steps in order: MPO application of each
-    site, then the QR left sweep over the window carrying R from site to site,
-    then the truncating SVD right sweep back over it carrying S V) and scale
-    to the sweep by the closed-form per-site flop model."""
+def sweep_site_equivalents(N, d, chi, D=5):
+    """Bulk-site equivalents of the reference sweep on an N-site chain with
+    bonds [1, chi, ..., chi, 1]: the per-site flop model summed over the
+    sites, with the QR sweep's left bonds min(d * previous, chi D) and the
+    truncated right bonds min(chi, d^(N-1-i)), divided by a bulk site's."""
+    qb = [1]
+    for i in range(N - 1):
+        qb.append(min(d * qb[-1], chi * D))
+    bulk = flops_per_site_plain(chi, chi, D, D, d, chi, chi * D)
+    return sum(flops_per_site_plain(chi, chi, D, D, d, min(chi, d ** (N - 1 - i)), qb[i]) / bulk
+               for i in range(1, N - 1))
+
+
+def cpu_baseline_sweep(args, N_small=22, d=2, D=5):
+    """The oracle's full reference sweep (contract_mps_mpo, then
+    svd_compress(chi, 1e-15)) of an N_small-site chain at the workload's chi,
+    timed on the host cores and scaled to the N-site sweep by the per-site
+    flop model; its result also checks the GPU paths on the same chain
+    (state error ||psi - psi_oracle||^2 / ||psi_oracle||^2, bond dimensions).
+    Returns (cpu_baseline record, oracle check record)."""
     import numpy as np
+    import torch
+    import tncontract_amd as tn
+    from tncontract_amd.onedim import random_mps_arrays, xxz_mpo
+    from tncontract_amd.onedim.structured import apply_mpo_compress, SweepStats
     from oracle import tn_oracle as o
     from oracle import models as om
     try:
@@ -136,61 +156,45 @@ def cpu_baseline(N, d, chi, D=5, window=4):
     except Exception:  # pragma: no cover
         blas = []
     cores = len(os.sched_getaffinity(0))
-    rng = np.random.default_rng(1)
-    chiD = chi * D
-    W = o.OTensor(om.xxz_bulk_w(), ["left", "right", "physout", "physin"])
-    Bs = [o.OTensor(rng.standard_normal((d, chi, chi)) / np.sqrt(2 * chi),
-                    ["phys", "left", "right"]) for _ in range(window)]
-    # R of the QR sweep entering the window (upper triangular, chi D x chi D)
-    R = o.OTensor(np.triu(rng.standard_normal((chiD, chiD))), ["qr_out", "left"])
+    chi = args.chi
+    arrs = random_mps_arrays(N_small, d, chi, seed=args.seed)
     t0 = time.perf_counter()
-    sites = []
-    for B in Bs:                                                       # MPO application
-        A = o.contract(B, W, "phys", "physin")
-        A.consolidate_indices()                                        # (left, physout, right)
-        sites.append(A)
+    r = o.contract_mps_mpo(o.OMPS([o.OTensor(a, ["phys", "left", "right"]) for a in arrs]),
+                           om.xxz_mpo(N_small))
     t_mpo = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for i in range(window):                                            # QR left sweep
-        nxt = o.contract(R, sites[i], "left", "left")
-        nxt.replace_label("qr_out", "left")
-        Q, R = o.tensor_qr(nxt, ["physout", "left"], qr_label="qr_")
-        Q.replace_label("qr_in", "right")
-        sites[i] = Q
-        R.replace_label("right", "left")
-    t_qr = time.perf_counter() - t0
-    # right sweep: the window's last site already carries the truncated
-    # chi-bond part of the sites to its right
-    SV = o.OTensor(np.linalg.qr(rng.standard_normal((chiD, chi)))[0], ["left", "right"])
-    t0 = time.perf_counter()
-    for i in range(window - 1, -1, -1):                                # truncating SVD sweep
-        c = o.contract(sites[i], SV, "right", "left")                  # (left, physout, right=chi)
-        U, S, V = o.tensor_svd(c, ["physout", "right"])
-        U.data = U.data[..., :chi]
-        S.data = S.data[:chi, :chi]
-        V.data = V.data[:chi]
-        SV = o.contract(S, V, "svd_in", "svd_out")                     # (svd_out, left)
-        SV.replace_label(["svd_out", "left"], ["right", "left"])
-        SV.data = SV.data.T.copy()
-        SV.labels = ["left", "right"]
-    t_svd = time.perf_counter() - t0
-    t_win = t_mpo + t_qr + t_svd
-    # scale by the per-site flop model over the sweep's actual shapes
-    qb = [1]  # left bond after the QR sweep: min(d * previous, chi * D)
-    for i in range(N - 1):
-        qb.append(min(d * qb[-1], chiD))
-    bulk = flops_per_site_plain(chi, chi, D, D, d, chi, chiD)
-    total = 0.0
-    for i in range(1, N - 1):
-        total += flops_per_site_plain(chi, chi, D, D, d, chi, qb[i]) / bulk
-    sweep_s = t_win / window * max(total, 1.0)
-    return {"value": 1.0 / sweep_s, "unit": "sweeps/s", "cores": cores, "kind": "port",
-            "sample": ("NumPy FP64 oracle, %d consecutive bulk sites of the reference sweep at "
-                       "chi=%d, D=%d (MPO application %.2fs, QR left sweep with R carried %.2fs, "
-                       "truncating SVD right sweep %.2fs; %.2fs per site), scaled to the %d-site "
-                       "sweep by the closed-form per-site flop model (%.1f bulk-site "
-                       "equivalents); BLAS %s"
-                       % (window, chi, D, t_mpo, t_qr, t_svd, t_win / window, N, total, blas))}
+    r.svd_compress(chi=chi, threshold=1e-15)
+    t_all = time.perf_counter() - t0
+    scale = sweep_site_equivalents(args.N, d, chi, D) / sweep_site_equivalents(N_small, d, chi, D)
+    cpu = {"value": 1.0 / (t_all * scale), "unit": "sweeps/s", "cores": cores, "kind": "port",
+           "sample": ("NumPy FP64 oracle, the full reference sweep (contract_mps_mpo %.2fs, then "
+                      "svd_compress(chi=%d, 1e-15): QR left sweep + truncating SVD right sweep) "
+                      "of a %d-site chain of the workload (%.2fs), scaled to the %d-site sweep "
+                      "by the closed-form per-site flop model (x%.2f); BLAS %s"
+                      % (t_mpo, chi, N_small, t_all, args.N, scale, blas))}
+
+    def to_oracle(g):
+        return o.OMPS([o.OTensor(t.numpy().astype(np.float64), t.labels) for t in g],
+                      g.left_label, g.right_label, g.phys_label)
+
+    def err(g):
+        a = to_oracle(g)
+        nb = o.inner_product_mps(r, r)
+        return float((o.inner_product_mps(a, a) + nb - 2 * o.inner_product_mps(a, r)) / nb)
+
+    psi = tn.onedim.MatrixProductState([tn.Tensor(a, ["phys", "left", "right"]) for a in arrs])
+    st = SweepStats()
+    fast = apply_mpo_compress(psi, xxz_mpo(N_small), chi=chi, threshold=1e-15,
+                              precision=args.precision, stats=st, eig_tol=args.eig_tol)
+    plain = tn.onedim.contract_mps_mpo(psi, xxz_mpo(N_small))
+    plain.svd_compress(chi=chi, threshold=1e-15)
+    torch.cuda.synchronize()
+    check = {"chain": "XXZ MPO D=5 x random MPS N=%d d=%d chi=%d -> chi=%d (seed %d)"
+                      % (N_small, d, chi, chi, args.seed),
+             "fast_state_err": err(fast), "fast_bonddims_equal": fast.bonddims() == r.bonddims(),
+             "fast_fallback_bond": st.fallback,
+             "plain_fp64_state_err": err(plain),
+             "plain_bonddims_equal": plain.bonddims() == r.bonddims()}
+    return cpu, check
 
 
 # FP64-equivalent peaks (TFLOP/s) of the tagged kernels for the sweep-level
@@ -941,10 +945,11 @@ def chain_workload(args):
         if res_fast.bonddims() != res_plain.bonddims():
             err = float("nan")
 
-    cpu = None
+    cpu, check = None, None
     if not args.no_cpu:
-        log("cpu baseline sample")
-        cpu = cpu_baseline(N, d, chi)
+        log("cpu baseline: the oracle's full sweep of a 22-site chain, and the GPU paths "
+            "checked against it")
+        cpu, check = cpu_baseline_sweep(args)
 
     rec = {
         "metric": METRIC,
@@ -977,6 +982,7 @@ def chain_workload(args):
         "sweep_roofline": sweep_roof,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "oracle_check": check,
     }
     if not args.no_extra:
         ex = argparse.Namespace(**vars(args))
