@@ -11,7 +11,10 @@
 //    per tile product: hi*hi + hi*lo + lo*hi, FP32 accumulation).
 //  * contract_direct: one thread per output element, iterated in the
 //    output's memory order; used when K is small (e.g. the K = d = 2 MPO
-//    application), where the work is HBM-bound on the stores.
+//    application), where the work is HBM-bound on the stores.  With 32-bit
+//    offsets and K <= 8 (the usual case) contract_direct32 decodes the
+//    output index by magic-number division and reads the contracted offsets
+//    from a host-built table.
 #include "tn_internal.h"
 #include "tn_prof.h"
 
@@ -433,6 +436,68 @@ __global__ void __launch_bounds__(256) contract_direct(DirectArgs g) {
   }
 }
 
+// Division by a runtime constant d >= 1 for 0 <= n < 2^31: q = (mulhi(n, m) +
+// n) >> l with l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1 (the
+// round-up multiplier method of Granlund and Montgomery, PLDI 1994; the sum
+// stays below 2^32 because n < 2^31).  Exact for every such n (checked on the
+// host against integer division over the whole range of d).
+struct FastDiv {
+  uint32_t d, m, l;
+};
+
+static FastDiv make_fastdiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const uint64_t m = (((1ull << 32) * ((1ull << l) - d)) / d) + 1;
+  return FastDiv{d, (uint32_t)m, l};
+}
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, const FastDiv& f) {
+  return (__umulhi(n, f.m) + n) >> f.l;
+}
+
+// contract_direct with 32-bit offsets (every offset of A, B and C and the
+// output count below 2^31), magic-number index decoding and the K <= 8
+// contracted offsets precomputed on the host: the MPO application's
+// per-element work is then a few integer multiply-adds and K fused
+// multiply-adds, and the kernel is bound by its stores.
+struct Direct32Args {
+  const void* A;
+  const void* B;
+  void* C;
+  int no, K;
+  FastDiv od[2 * TN_MAX_MODES];
+  int32_t oa[2 * TN_MAX_MODES], ob[2 * TN_MAX_MODES], oc[2 * TN_MAX_MODES];
+  int32_t ka[8], kb[8];
+  uint32_t nout;
+  double alpha, beta;
+};
+
+template <typename T, typename Acc>
+__global__ void __launch_bounds__(256) contract_direct32(Direct32Args g) {
+  const T* __restrict__ A = (const T*)g.A;
+  const T* __restrict__ B = (const T*)g.B;
+  T* __restrict__ C = (T*)g.C;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < g.nout; o += stride) {
+    uint32_t idx = o;
+    int32_t pa = 0, pb = 0, pc = 0;
+    for (int i = g.no - 1; i >= 0; --i) {
+      const uint32_t q = fast_div(idx, g.od[i]);
+      const int32_t r = (int32_t)(idx - q * g.od[i].d);
+      pa += r * g.oa[i];
+      pb += r * g.ob[i];
+      pc += r * g.oc[i];
+      idx = q;
+    }
+    Acc acc = 0;
+    for (int k = 0; k < g.K; ++k) acc += (Acc)A[pa + g.ka[k]] * (Acc)B[pb + g.kb[k]];
+    T v = (T)(g.alpha * (double)acc);
+    if (g.beta != 0.0) v += (T)g.beta * C[pc];
+    C[pc] = v;
+  }
+}
+
 // Merge adjacent modes of a group when the merge is valid for every operand
 // that shares the group (strides of the slower mode = stride*dim of the faster).
 static void merge_group(std::vector<int64_t>& dims, std::vector<std::vector<int64_t>*> strides) {
@@ -575,6 +640,50 @@ int contract_launch(const void* A, const Modes& a_m, const Modes& a_k, const voi
     unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(g.nout, 256), 65536 * 4);
     ProfScope prof(PROF_DIRECT, st, 2.0 * (double)g.nout * (double)K,
                    (double)(dtype == TN_F64 ? 8 : 4) * ((double)g.nout * 2.0));
+    // 32-bit variant: every offset below 2^31 and K <= 8
+    auto span = [](const int64_t* d, const int64_t* st_, int n) {
+      int64_t m = 0;
+      for (int i = 0; i < n; ++i) m += (d[i] - 1) * (st_[i] < 0 ? -st_[i] : st_[i]);
+      return m;
+    };
+    const int64_t lim = (int64_t)1 << 31;
+    bool fits = g.nout < lim && K >= 1 && K <= 8;
+    for (int i = 0; i < g.no && fits; ++i)
+      fits = g.oa[i] >= 0 && g.ob[i] >= 0 && g.oc[i] >= 0;
+    for (int i = 0; i < g.nk && fits; ++i) fits = g.ka[i] >= 0 && g.kb[i] >= 0;
+    fits = fits && span(g.odim, g.oa, g.no) + span(g.kdim, g.ka, g.nk) < lim &&
+           span(g.odim, g.ob, g.no) + span(g.kdim, g.kb, g.nk) < lim &&
+           span(g.odim, g.oc, g.no) < lim;
+    if (fits) {
+      Direct32Args h{};
+      h.A = A; h.B = B; h.C = C;
+      h.no = g.no;
+      h.K = (int)K;
+      for (int i = 0; i < g.no; ++i) {
+        h.od[i] = make_fastdiv((uint32_t)g.odim[i]);
+        h.oa[i] = (int32_t)g.oa[i]; h.ob[i] = (int32_t)g.ob[i]; h.oc[i] = (int32_t)g.oc[i];
+      }
+      for (int64_t k = 0; k < K; ++k) {   // contracted offsets, mode 0 slowest
+        int64_t kidx = k, qa = 0, qb = 0;
+        for (int i = g.nk - 1; i >= 0; --i) {
+          const int64_t r = kidx % g.kdim[i];
+          qa += r * g.ka[i];
+          qb += r * g.kb[i];
+          kidx /= g.kdim[i];
+        }
+        h.ka[k] = (int32_t)qa;
+        h.kb[k] = (int32_t)qb;
+      }
+      h.nout = (uint32_t)g.nout;
+      h.alpha = alpha;
+      h.beta = beta;
+      if (dtype == TN_F64)
+        hipLaunchKernelGGL((contract_direct32<double, double>), dim3(blocks), dim3(256), 0, st, h);
+      else
+        hipLaunchKernelGGL((contract_direct32<float, float>), dim3(blocks), dim3(256), 0, st, h);
+      TN_CHECK_LAUNCH();
+      return TN_OK;
+    }
     if (dtype == TN_F64)
       hipLaunchKernelGGL((contract_direct<double, double>), dim3(blocks), dim3(256), 0, st, g);
     else
