@@ -254,16 +254,20 @@ def batched_workload(args):
     cpu = None
     if not args.no_cpu:
         from oracle import tn_oracle as o
-        sample = jobs[:16]
+        # the whole batch (about 20 s on 16 host cores), capped at 60 s of work
         t1 = time.perf_counter()
-        for Lm, Rm, _ in sample:
+        done = 0
+        for Lm, Rm, _ in jobs:
             A = o.OTensor(Lm.astype(np.float64), ["l", "m"])
             B = o.OTensor(Rm.astype(np.float64), ["m", "r"])
             o.truncated_svd(o.contract(A, B, "m", "m"), ["l"], threshold=1e-8, absolute=False)
-        tc = (time.perf_counter() - t1) / len(sample)
-        cpu = {"value": 1.0 / tc, "unit": "jobs/s", "cores": len(os.sched_getaffinity(0)),
-               "kind": "port", "sample": "NumPy FP64 oracle contract + truncated_svd on the "
-                                         "first 16 jobs (%.2f s)" % (tc * len(sample))}
+            done += 1
+            if time.perf_counter() - t1 > 60.0:
+                break
+        tt = time.perf_counter() - t1
+        cpu = {"value": done / tt, "unit": "jobs/s", "cores": len(os.sched_getaffinity(0)),
+               "kind": "port", "sample": "NumPy FP64 oracle contract + truncated_svd of %d of the "
+                                         "%d jobs, in order (%.2f s)" % (done, len(jobs), tt)}
     L.prof_collect("gemm_f64", reset=True)
     L.prof_enable(True)
     batched_truncate(Ls, Rs, rel_threshold=1e-8)
