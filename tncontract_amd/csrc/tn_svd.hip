@@ -32,8 +32,15 @@
 
 namespace tn {
 
-constexpr int JNT = 2;                // row tiles per wave in jac_eig_apply's apply
-constexpr int JCH = 64 * JNT;         // rows per chunk of jac_eig_apply
+// Rows per chunk of jac_eig_apply: 64 NT with NT = 2 (128 rows), or NT = 4
+// (256 rows) when 128-row chunks would put more than two blocks on a CU:
+// the blocks' rotation rounds then share the SIMDs' issue slots and stop
+// being latency-bound (n = 2048: 64 pairs x 8 chunks of 256 instead of 16
+// chunks of 128).
+constexpr int kEigResident = 512;
+static int eig_nt(int npairs, int64_t L, int64_t kz) {
+  return (int64_t)npairs * (ceil_div(L, 128) + ceil_div(kz, 128)) <= kEigResident ? 2 : 4;
+}
 constexpr int GCH = 128;     // rows per Gram partial of jac_gram (twice the blocks)
 
 // Partial Gram P^T P of the pair's 32 vectors over rows [r0, r1) into out
@@ -145,11 +152,13 @@ __global__ void __launch_bounds__(256) jac_gram(const double* __restrict__ Wv, i
 // own row chunk by MFMA.  Computing the 32 x 32 eigensolve redundantly per
 // chunk costs nothing in latency and saves a launch and the Q round trip
 // through memory.
+template <int NT>
 __global__ void __launch_bounds__(256) jac_eig_apply(double* __restrict__ Wv, int64_t L,
                                                      double* __restrict__ Z, int64_t nv, int s,
                                                      int nblk, int nchW, int nchG,
                                                      const double* __restrict__ Gpart, double tol,
                                                      unsigned long long* __restrict__ offmax) {
+  constexpr int CH = 64 * NT;
   __shared__ EigLds sm;
   const int pi = blockIdx.x, ch = blockIdx.y;
   int I, J;
@@ -157,18 +166,18 @@ __global__ void __launch_bounds__(256) jac_eig_apply(double* __restrict__ Wv, in
   double* base;
   int64_t ld, rs, re;
   if (ch < nchW) {
-    base = Wv; ld = L; rs = (int64_t)ch * JCH; re = min(L, rs + JCH);
+    base = Wv; ld = L; rs = (int64_t)ch * CH; re = min(L, rs + CH);
   } else {
-    base = Z; ld = nv; rs = (int64_t)(ch - nchW) * JCH; re = min(nv, rs + JCH);
+    base = Z; ld = nv; rs = (int64_t)(ch - nchW) * CH; re = min(nv, rs + CH);
   }
   eig_load(sm, Gpart + (int64_t)pi * nchG * (J2 * J2), nchG);
-  double x[8][JNT];
-  apply_rows_fetch<JNT>(base, ld, nv, rs, re, I, J, x);   // in flight during the rounds
+  double x[8][NT];
+  apply_rows_fetch<NT>(base, ld, nv, rs, re, I, J, x);   // in flight during the rounds
   const double off = eig_off(sm, tol);
   if (ch == 0 && threadIdx.x == 0) atomicMax(offmax, dbits(off));
   if (off <= tol) return;
   const int cur = eig_rotate(sm, tol);
-  apply_rows_finish<JNT>(base, ld, nv, rs, re, I, J, x, sm.Q[cur]);
+  apply_rows_finish<NT>(base, ld, nv, rs, re, I, J, x, sm.Q[cur]);
 }
 
 // sigma_j = ||Wv_j|| (one block per vector)
@@ -362,7 +371,9 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
   if (nblk % 2) nblk += 1;
   const int npairs = nblk / 2;
   // Z == nullptr: the rotations are not accumulated (eigensolver use)
-  const int nchW = (int)ceil_div(L, JCH), nchZ = Z ? (int)ceil_div(k, JCH) : 0;
+  const int64_t kz0 = Z ? k : 0;
+  const int ent = eig_nt(npairs, L, kz0);
+  const int nchW = (int)ceil_div(L, 64 * ent), nchZ = Z ? (int)ceil_div(k, 64 * ent) : 0;
   const int nchG = (int)ceil_div(L, GCH);
   const int64_t kz = Z ? k : 0;
   if (Z)
@@ -403,8 +414,12 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
         // chunks read and written
         ProfScope p(PROF_JAC_EIG, st, 31.0 * 9216.0 * npairs + apply_fl,
                     8.0 * (double)nchG * J2 * J2 * npairs + 16.0 * (double)(L + kz) * J2 * npairs);
-        hipLaunchKernelGGL(jac_eig_apply, dim3(npairs, nchW + nchZ), dim3(256), 0, st, Wv, L, Z,
-                           k, s, nblk, nchW, nchG, w.Gpart, tol, w.off);
+        if (ent == 2)
+          hipLaunchKernelGGL(jac_eig_apply<2>, dim3(npairs, nchW + nchZ), dim3(256), 0, st, Wv,
+                             L, Z, k, s, nblk, nchW, nchG, w.Gpart, tol, w.off);
+        else
+          hipLaunchKernelGGL(jac_eig_apply<4>, dim3(npairs, nchW + nchZ), dim3(256), 0, st, Wv,
+                             L, Z, k, s, nblk, nchW, nchG, w.Gpart, tol, w.off);
       }
     }
     TN_CHECK_LAUNCH();
