@@ -64,6 +64,10 @@ int tn_convert(const void* src, int src_dtype, void* dst, int dst_dtype, int64_t
 /* x *= alpha (tensor.py:95-123 scalar multiply; onedim_core.py:358, 482). */
 int tn_scale(void* x, int dtype, int64_t n, double alpha, void* stream);
 
+/* x = sqrt(x) elementwise (np.sqrt of the singular values when tensor_svd
+ * absorbs them into both factors, tensor.py:950-956). */
+int tn_sqrt(void* x, int dtype, int64_t n, void* stream);
+
 /* out = a*x + b*y elementwise (Tensor.__add__, tensor.py:125-140; distance,
  * tensor.py:782-802). */
 int tn_axpby(const void* x, const void* y, void* out, int dtype, int64_t n, double a, double b,

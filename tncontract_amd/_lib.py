@@ -72,6 +72,7 @@ def lib():
         "tn_permute": (i32, [vp, mp, vp, i32, vp]),
         "tn_convert": (i32, [vp, i32, vp, i32, i64, vp]),
         "tn_scale": (i32, [vp, i32, i64, dbl, vp]),
+        "tn_sqrt": (i32, [vp, i32, i64, vp]),
         "tn_axpby": (i32, [vp, vp, vp, i32, i64, dbl, dbl, vp]),
         "tn_diag_embed": (i32, [vp, i64, vp, i32, vp]),
         "tn_diag_inverse_workspace": (sz, [i64]),
@@ -298,6 +299,13 @@ def scale_(t, alpha):
     """In-place t *= alpha; t must be contiguous."""
     if t.numel():
         check(lib().tn_scale(ptr(t), dtype_code(t), t.numel(), float(alpha), stream_ptr()))
+    return t
+
+
+def sqrt_(t):
+    """In-place t = sqrt(t); t must be contiguous."""
+    if t.numel():
+        check(lib().tn_sqrt(ptr(t), dtype_code(t), t.numel(), stream_ptr()))
     return t
 
 

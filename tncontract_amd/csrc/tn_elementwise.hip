@@ -130,6 +130,13 @@ __global__ void scale_kernel(T* __restrict__ x, int64_t n, T alpha) {
   for (; i < n; i += stride) x[i] *= alpha;
 }
 
+template <typename T>
+__global__ void sqrt_kernel(T* __restrict__ x, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) x[i] = sqrt(x[i]);
+}
+
 // Two-pass FP64 sum of squares: per-block partials, then one block sums them.
 template <typename T>
 __global__ void __launch_bounds__(256) sumsq_partial(const T* __restrict__ x, int64_t n,
@@ -338,6 +345,23 @@ int tn_convert(const void* src, int src_dtype, void* dst, int dst_dtype, int64_t
                        (const float*)src, (float*)dst, n);
   else {
     set_error(TN_ERR_UNSUPPORTED, "tn_convert: unsupported dtypes %d -> %d", src_dtype, dst_dtype);
+    return TN_ERR_UNSUPPORTED;
+  }
+  TN_CHECK_LAUNCH();
+  return TN_OK;
+}
+
+int tn_sqrt(void* x, int dtype, int64_t n, void* stream) {
+  if (n == 0) return TN_OK;
+  TN_REQUIRE(n > 0 && x, "tn_sqrt: bad arguments");
+  hipStream_t st = as_stream(stream);
+  unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 256), 65536);
+  if (dtype == TN_F64)
+    hipLaunchKernelGGL(sqrt_kernel<double>, dim3(blocks), dim3(256), 0, st, (double*)x, n);
+  else if (dtype == TN_F32)
+    hipLaunchKernelGGL(sqrt_kernel<float>, dim3(blocks), dim3(256), 0, st, (float*)x, n);
+  else {
+    set_error(TN_ERR_UNSUPPORTED, "tn_sqrt: unsupported dtype %d", dtype);
     return TN_ERR_UNSUPPORTED;
   }
   TN_CHECK_LAUNCH();

@@ -385,6 +385,9 @@ static int jacobi_run(double* Wv, int64_t L, double* Z, int64_t k, const SvdWs& 
   const double gram_fl = 2.0 * (double)L * J2 * J2 * npairs;
   const double apply_fl = 2.0 * (double)(L + kz) * J2 * J2 * npairs;
   unsigned long long host_off = 0;
+  // the sort's six launches per sweep pay off only where a sweep is long
+  // (and graph-replayed): at least 16 steps, k >= 256
+  if (nblk - 1 < 16) sort_tmp = nullptr;
   auto enqueue_sweep = [&]() -> int {
     hipLaunchKernelGGL(reset_u64, dim3(1), dim3(1), 0, st, w.off);
     if (sort_tmp) {

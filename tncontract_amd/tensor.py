@@ -356,10 +356,9 @@ class Tensor:
             return
         src = d if d.dtype == torch.float64 else L.convert(L.contiguous(d), torch.float64)
         u, s, vt = _svd_device(src, 1)
-        sv = s.cpu().numpy()
-        if sv[-1] == 0.0:
+        w, winfo = L.diag_inverse(L.diag_embed(s))
+        if winfo[1] > 0:
             raise np.linalg.LinAlgError("Singular matrix")
-        w = L.diag_embed(_to_device(np.ascontiguousarray(1.0 / sv, dtype=np.float64)))
         inv = _contract_data(vt, [0], _contract_data(w, [1], u, [1]), [0])
         self._data = inv if d.dtype == torch.float64 else L.convert(inv, d.dtype)
 
@@ -656,8 +655,9 @@ def _absorb(U, S, V, how):
 
 
 def _sqrt_diag(Sdata):
-    s = L.convert(torch.diagonal(Sdata), torch.float64)
-    s = torch.from_numpy(np.sqrt(np.maximum(s.cpu().numpy(), 0.0))).to(Sdata.device)
+    d = torch.diagonal(Sdata)
+    # a fresh copy (materialize / convert): the square root is taken in place
+    s = L.sqrt_(L.materialize(d) if d.dtype == torch.float64 else L.convert(d, torch.float64))
     return L.diag_embed(s, Sdata.dtype)
 
 
