@@ -5,9 +5,10 @@ keep rule, so a randomized range-finder SVD
 iterations with Householder re-orthogonalisation) can replace the full
 Jacobi SVD in the truncating sweep of svd_compress.  The sketch is accepted
 only when it certifies the reference's keep rule (sigma / sigma_0 >
-threshold): the spectral norm of the residual A - U S V^T, estimated by
-block power iteration and multiplied by a safety factor, must not exceed
-threshold * sigma_0 (so every singular value the sketch missed lies below
+threshold): an upper bound on the spectral norm of the residual
+A - U S V^T, holding with probability at least 1 - 1e-10 (the posterior
+estimate from 20 Gaussian test vectors, Halko, Martinsson & Tropp, Lemma
+4.1), must not exceed threshold * sigma_0 (so every singular value the sketch missed lies below
 the threshold), and no sketched value may lie within that bound below the
 threshold (so no kept value can hide under the estimate).  Otherwise the full
 one-sided Jacobi SVD runs (the fallback), exactly as in svd_compress.
@@ -312,8 +313,8 @@ def learned_svd_compress(mps, chi, threshold, model, prev_spectra, stats=None, m
     return out
 
 
-# A sketch certifies the keep rule when its residual (times the safety
-# factor) is below threshold * sigma_0, i.e. when it holds every singular
+# A sketch certifies the keep rule when its residual bound is below
+# threshold * sigma_0, i.e. when it holds every singular
 # value down to about threshold / CERT_FACTOR: the rank the model predicts
 # (the sketch's size less its oversampling) is that count, not the kept rank.
 CERT_FACTOR = 16.0
