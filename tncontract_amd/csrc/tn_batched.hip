@@ -23,8 +23,15 @@
 //    rank-deficient product lie below the keep threshold whatever their
 //    orthogonality, and orthogonalising that cluster was most of the sweeps
 //    of such jobs.
+//    Truncation jobs with 32 <= k and L <= 1024 are preconditioned by a
+//    Householder QR (tn_bqr.h, one workgroup per job): the Jacobi runs on
+//    the k rows of R (vectors of length k) and the left vectors are Q Z.
+//    On the graded, rank-deficient products of the truncation workload this
+//    takes ~8-11 sweeps where the columns of the product take ~20-40
+//    (simulated on the workload's spectra).
 //  * per-job sort, rank selection (sigma/sigma_0 > threshold, <= chi) and
 //    output of U, S, V^T.
+#include "tn_bqr.h"
 #include "tn_internal.h"
 #include "tn_jacobi.h"
 #include "tn_prof.h"
@@ -160,7 +167,8 @@ constexpr int BANT = 2, BACH = 64 * BANT;   // row tiles per wave / rows per chu
 static_assert(BJB == JB && BJ2 == J2, "batched pairs use the shared pair layout");
 
 struct BJob {
-  int64_t wv;     // offset of the job's vectors (k vectors of length L)
+  int64_t wv;     // offset of the job's Jacobi vectors (k vectors of length L: the
+                  // columns or rows of its matrix, or the rows of R)
   int64_t z;      // offset of the job's rotation accumulator (k x k)
   int32_t L, k, nblk, nch;
   int32_t ncha, nchaz;   // row chunks of W and Z in bjac_apply (BACH rows)
@@ -524,6 +532,8 @@ struct OJob {
   void* U;     // m x k row-major (FP32 or FP64 output)
   void* Vt;    // k x n row-major
   double* S;   // k
+  const double* qz;  // QR-preconditioned jobs: Q Z, the left vectors of the
+                     // job's vector matrix in sorted order (L x kept); else null
   int32_t m, n;
   int32_t tall;
   int32_t f64;  // outputs are FP64
@@ -549,10 +559,15 @@ __global__ void bgather_f64(double* __restrict__ Wv, const BJob* __restrict__ jo
   }
 }
 
+// Vector matrix X (L x k: the columns of the job's matrix if tall, its rows
+// otherwise) = U_X S V_X^T.  Plain jobs: U_X = the rotated vectors / S, V_X =
+// Z.  QR-preconditioned jobs (X = Q R, Jacobi on the rows of R): U_X = Q Z
+// (O.qz, columns c < kept in sorted order; zero beyond), V_X = the rotated
+// rows of R / S.  Tall: U = U_X, V^T = V_X^T; wide: U = V_X, V^T = U_X^T.
 __global__ void bout(const double* __restrict__ Wv, const double* __restrict__ Zb,
                      const BJob* __restrict__ jobs, const OJob* __restrict__ outs,
                      const int64_t* __restrict__ sbase, const int* __restrict__ perm,
-                     const double* __restrict__ Ss) {
+                     const double* __restrict__ Ss, const int* __restrict__ kept) {
   const int j = blockIdx.y;
   const BJob J = jobs[j];
   const OJob O = outs[j];
@@ -561,6 +576,29 @@ __global__ void bout(const double* __restrict__ Wv, const double* __restrict__ Z
   const int64_t tot = (int64_t)O.m * k + k * O.n;
   const int* pm = perm + sbase[j];
   const double* S = Ss + sbase[j];
+  if (O.qz) {
+    const int64_t L = O.tall ? O.m : O.n, kp = kept[j];
+    for (; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+      double v;
+      if (idx < (int64_t)O.m * k) {
+        const int64_t i = idx / k, c = idx - i * k;
+        if (O.tall) v = c < kp ? O.qz[c * L + i] : 0.0;
+        else v = S[c] > 0.0 ? Wv[J.wv + (int64_t)pm[c] * k + i] / S[c] : 0.0;
+        if (O.f64) ((double*)O.U)[idx] = v;
+        else ((float*)O.U)[idx] = (float)v;
+      } else {
+        const int64_t e = idx - (int64_t)O.m * k;
+        const int64_t r = e / O.n, c = e - r * O.n;
+        if (O.tall) v = S[r] > 0.0 ? Wv[J.wv + (int64_t)pm[r] * k + c] / S[r] : 0.0;
+        else v = r < kp ? O.qz[r * L + c] : 0.0;
+        if (O.f64) ((double*)O.Vt)[e] = v;
+        else ((float*)O.Vt)[e] = (float)v;
+      }
+    }
+    if (blockIdx.x == 0)
+      for (int64_t c = threadIdx.x; c < k; c += blockDim.x) O.S[c] = S[c];
+    return;
+  }
   for (; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
     if (idx < (int64_t)O.m * k) {
       int64_t i = idx / k, c = idx - i * k;
@@ -633,25 +671,57 @@ static double time_grouped(int FR, const std::vector<int>& ids, const std::vecto
 }
 
 
-static size_t batched_ws_bytes(const std::vector<int64_t>& ms, const std::vector<int64_t>& ns) {
+// QR preconditioning (tn_bqr.h) for the truncation jobs whose rows fit a
+// workgroup's registers and that have more than one block pair
+static bool qr_precond(bool trunc, int64_t L, int64_t k) {
+  return trunc && k > BJ2 && L <= kQrRows;
+}
+
+// Work items of a preconditioned job with k columns: one panel item per
+// panel; trailing-update groups of 64 columns after each panel; Q-apply
+// groups of 64 of the (at most k) kept columns per panel.
+static void qr_item_counts(int64_t k, size_t* npan, size_t* ntrail, size_t* napply) {
+  const int64_t np = ceil_div(k, QNB);
+  *npan += (size_t)np;
+  for (int64_t p = 0; p < np; ++p) *ntrail += (size_t)ceil_div(std::max<int64_t>(k - (p + 1) * QNB, 0), 64);
+  *napply += (size_t)(np * ceil_div(k, 64));
+}
+
+static size_t batched_ws_bytes(const std::vector<int64_t>& ms, const std::vector<int64_t>& ns,
+                               bool trunc) {
   const int njobs = (int)ms.size();
   if (njobs <= 0) return 256;
-  size_t wv = 0, z = 0, gi = 0, ei = 0, ai = 0, sv = 0;
+  size_t wv = 0, z = 0, gi = 0, ei = 0, ai = 0, sv = 0, qz = 0, tf = 0, npre = 0;
+  size_t npan = 0, ntrail = 0, napply = 0;
   for (int j = 0; j < njobs; ++j) {
     int64_t k = std::min(ms[j], ns[j]), L = std::max(ms[j], ns[j]);
+    const bool pre = qr_precond(trunc, L, k);
+    const int64_t Lj = pre ? k : L;  // Jacobi vector length
     int nblk = (int)ceil_div(std::max<int64_t>(k, 1), BJB);
     nblk += nblk % 2;
     int np = nblk / 2;
-    wv += (size_t)(L * k);
+    wv += (size_t)(L * k) + (pre ? (size_t)(k * k) : 0);
     z += (size_t)(k * k);
-    gi += (size_t)np * ceil_div(L, BCH);
+    gi += (size_t)np * ceil_div(Lj, BCH);
     ei += (size_t)np;
-    ai += (size_t)np * (ceil_div(L, BACH) + ceil_div(k, BACH));
+    ai += (size_t)np * (ceil_div(Lj, BACH) + ceil_div(k, BACH));
     sv += (size_t)k;
+    if (pre) {
+      qz += (size_t)(L * k);
+      tf += (size_t)ceil_div(k, QNB) * QNB * QNB;
+      ++npre;
+      qr_item_counts(k, &npan, &ntrail, &napply);
+    }
   }
   Arena a{nullptr, 0, 0};
   a.take<double>(wv);
   a.take<double>(z);
+  a.take<double>(qz);
+  a.take<double>(tf);
+  a.take<QJob>(npre);
+  a.take<int>(npan);
+  a.take<QItem>(ntrail);
+  a.take<QItem>(napply);
   a.take<double>(gi * BJ2 * BJ2);
   a.take<double>(ei * BJ2 * BJ2);
   a.take<int>(ei);
@@ -686,7 +756,8 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
     ms[j] = tj ? tj[j].m : sj[j].m;
     ns[j] = tj ? tj[j].n : sj[j].n;
   }
-  TN_REQUIRE(ws && ws_bytes >= batched_ws_bytes(ms, ns), "%s: workspace too small", who);
+  TN_REQUIRE(ws && ws_bytes >= batched_ws_bytes(ms, ns, tj != nullptr), "%s: workspace too small",
+             who);
   if (max_sweeps <= 0) max_sweeps = 40;
   hipStream_t st = as_stream(stream);
   // ---- host tables
@@ -694,8 +765,9 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   std::vector<OJob> oj(njobs);
   std::vector<GJob> gj(njobs);
   std::vector<int> g_job, g_pair, g_chunk, e_job, e_pair, a_job, a_pair, a_chunk;
-  std::vector<int64_t> sbase(njobs);
-  int64_t wv = 0, z = 0, sv = 0;
+  std::vector<int64_t> sbase(njobs), xoff(njobs);
+  std::vector<QJob> qj;
+  int64_t wv = 0, z = 0, sv = 0, qzs = 0, tfs = 0, rvs = 0;
   int maxsteps = 1, maxk = 1;
   std::vector<AJob> aj(njobs);
   for (int j = 0; j < njobs; ++j) {
@@ -718,15 +790,18 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
     TN_REQUIRE(J.m <= (1 << 20) && J.n <= (1 << 20) && J.k <= (1 << 20), "%s: job %d too large",
                who, j);
     const int64_t k = std::min(J.m, J.n), L = std::max(J.m, J.n);
+    const bool pre = qr_precond(tj != nullptr, L, k);
+    const int64_t Lj = pre ? k : L;  // Jacobi vector length
     BJob b{};
-    b.wv = wv;
+    xoff[j] = wv;
+    b.wv = wv;  // preconditioned: moved to the rows of R below
     b.z = z;
-    b.L = (int)L;
+    b.L = (int)Lj;
     b.k = (int)k;
     b.nblk = (int)ceil_div(k, BJB);
     b.nblk += b.nblk % 2;
-    b.nch = (int)ceil_div(L, BCH);
-    b.ncha = (int)ceil_div(L, BACH);
+    b.nch = (int)ceil_div(Lj, BCH);
+    b.ncha = (int)ceil_div(Lj, BACH);
     b.nchaz = (int)ceil_div(k, BACH);
     b.gram_base = (int)g_job.size();
     b.eig_base = (int)e_job.size();
@@ -736,8 +811,8 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
     // below the FP32 outputs' rounding, while the clusters of tiny singular
     // values that the FP32 rounding of the factors leaves (~1e-8 sigma_0)
     // would need up to twice the sweeps to reach L eps.
-    b.tol = tj ? std::max(1e-10, (double)L * 2.220446049250313e-16)
-               : (double)L * 2.220446049250313e-16;
+    b.tol = tj ? std::max(1e-10, (double)Lj * 2.220446049250313e-16)
+               : (double)Lj * 2.220446049250313e-16;
     b.neg2 = 0.0;
     for (int p = 0; p < b.nblk / 2; ++p) {
       for (int c = 0; c < b.nch; ++c) { g_job.push_back(j); g_pair.push_back(p); g_chunk.push_back(c); }
@@ -750,7 +825,22 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
     const bool tall = J.m >= J.n;
     gj[j] = GJob{(const float*)J.L, (const float*)J.R, nullptr, (int)J.m, (int)J.n, (int)J.k,
                  tall ? 1 : (int64_t)J.n, tall ? (int64_t)J.m : 1};
-    oj[j] = OJob{J.U, J.Vt, J.S, (int)J.m, (int)J.n, tall ? 1 : 0, tj ? 0 : 1};
+    oj[j] = OJob{J.U, J.Vt, J.S, nullptr, (int)J.m, (int)J.n, tall ? 1 : 0, tj ? 0 : 1};
+    if (pre) {
+      QJob q{};
+      q.x = wv;
+      q.r = rvs;  // relative to the R region, rebased below
+      q.qz = qzs;
+      q.tf = tfs;
+      q.z = z;
+      q.L = (int)L;
+      q.k = (int)k;
+      q.job = j;
+      qj.push_back(q);
+      rvs += k * k;
+      qzs += L * k;
+      tfs += ceil_div(k, QNB) * QNB * QNB;
+    }
     wv += L * k;
     z += k * k;
     sv += k;
@@ -758,10 +848,44 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
     maxk = std::max<int>(maxk, (int)k);
   }
   const int ngi = (int)g_job.size(), nei = (int)e_job.size(), nai = (int)a_job.size();
+  const int npre = (int)qj.size();
+  // the rows of R follow the matrices in the vector buffer
+  for (QJob& q : qj) {
+    q.r += wv;
+    bj[q.job].wv = q.r;
+  }
   // ---- device workspace
   Arena a{(char*)ws, ws_bytes, 0};
-  double* dWv = a.take<double>((size_t)wv);
+  double* dWv = a.take<double>((size_t)(wv + rvs));
   double* dZ = a.take<double>((size_t)z);
+  double* dQZ = a.take<double>((size_t)qzs);
+  double* dTf = a.take<double>((size_t)tfs);
+  QJob* dqj = a.take<QJob>((size_t)npre);
+  // per-panel work lists of the QR (panels in order) and of the Q application
+  int maxpan = 0;
+  for (const QJob& q : qj) maxpan = std::max<int>(maxpan, (int)ceil_div(q.k, QNB));
+  std::vector<int> pan_items;
+  std::vector<QItem> trail_items, apply_items;
+  std::vector<size_t> pan_off(maxpan + 1), trail_off(maxpan + 1), apply_off(maxpan + 1);
+  for (int p = 0; p < maxpan; ++p) {
+    pan_off[p] = pan_items.size();
+    trail_off[p] = trail_items.size();
+    apply_off[p] = apply_items.size();
+    for (int i = 0; i < npre; ++i) {
+      const int k = qj[i].k;
+      if (p * QNB >= k) continue;
+      pan_items.push_back(i);
+      const int ng = (int)ceil_div(std::max(k - (p + 1) * QNB, 0), 64);
+      for (int g = 0; g < ng; ++g) trail_items.push_back(QItem{i, g});
+      for (int g = 0; g < (int)ceil_div(k, 64); ++g) apply_items.push_back(QItem{i, g});
+    }
+  }
+  pan_off[maxpan] = pan_items.size();
+  trail_off[maxpan] = trail_items.size();
+  apply_off[maxpan] = apply_items.size();
+  int* dpan = a.take<int>(pan_items.size());
+  QItem* dtrail = a.take<QItem>(trail_items.size());
+  QItem* dapply = a.take<QItem>(apply_items.size());
   double* dG = a.take<double>((size_t)ngi * BJ2 * BJ2);
   double* dQ = a.take<double>((size_t)nei * BJ2 * BJ2);
   int* dskip = a.take<int>((size_t)nei);
@@ -786,7 +910,8 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   int64_t* dsbase = a.take<int64_t>((size_t)njobs);
   int* dgbase = a.take<int>(1);
   TN_REQUIRE(dgbase != nullptr, "%s: workspace too small", who);
-  for (int j = 0; j < njobs; ++j) gj[j].C = dWv + bj[j].wv;
+  for (int j = 0; j < njobs; ++j) gj[j].C = dWv + xoff[j];
+  for (const QJob& q : qj) oj[q.job].qz = dQZ + q.qz;
   std::vector<int> gt(ngi * 3), et(nei * 2), at(nai * 3);
   for (int i = 0; i < ngi; ++i) { gt[i] = g_job[i]; gt[ngi + i] = g_pair[i]; gt[2 * ngi + i] = g_chunk[i]; }
   for (int i = 0; i < nei; ++i) { et[i] = e_job[i]; et[nei + i] = e_pair[i]; }
@@ -799,6 +924,16 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   TN_HIP(hipMemcpyAsync(dgj, gj.data(), sizeof(GJob) * njobs, hipMemcpyHostToDevice, st));
   TN_HIP(hipMemcpyAsync(daj, aj.data(), sizeof(AJob) * njobs, hipMemcpyHostToDevice, st));
   TN_HIP(hipMemcpyAsync(dsbase, sbase.data(), sizeof(int64_t) * njobs, hipMemcpyHostToDevice, st));
+  if (npre) {
+    TN_HIP(hipMemcpyAsync(dqj, qj.data(), sizeof(QJob) * npre, hipMemcpyHostToDevice, st));
+    TN_HIP(hipMemcpyAsync(dpan, pan_items.data(), sizeof(int) * pan_items.size(),
+                          hipMemcpyHostToDevice, st));
+    if (!trail_items.empty())
+      TN_HIP(hipMemcpyAsync(dtrail, trail_items.data(), sizeof(QItem) * trail_items.size(),
+                            hipMemcpyHostToDevice, st));
+    TN_HIP(hipMemcpyAsync(dapply, apply_items.data(), sizeof(QItem) * apply_items.size(),
+                          hipMemcpyHostToDevice, st));
+  }
   TN_HIP(hipMemsetAsync(ddone, 0, sizeof(int) * njobs, st));
   TN_HIP(hipMemsetAsync(dsweeps, 0, sizeof(int) * njobs, st));
   TN_HIP(hipMemsetAsync(doff, 0, sizeof(unsigned long long) * njobs, st));
@@ -862,6 +997,27 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
       ProfScope p(PROF_GEMM_F64, st, work(use4), bytes(use4));
       TN_TRY(launch_grouped(4, use4, gj, dgj, dtile, dids, st));
     }
+  }
+  // ---- 1b. QR preconditioning: X = Q R in place, the rows of R as the
+  //          Jacobi vectors
+  if (npre) {
+    // Householder QR work 2 L k^2 - 2 k^3 / 3; each matrix read and written once
+    double qf = 0.0, qb = 0.0;
+    for (const QJob& q : qj) {
+      qf += 2.0 * q.L * (double)q.k * q.k - 2.0 / 3.0 * (double)q.k * q.k * q.k;
+      qb += 16.0 * q.L * (double)q.k;
+    }
+    ProfScope pr(PROF_QR_PANEL, st, qf, qb);
+    for (int p = 0; p < maxpan; ++p) {
+      const unsigned np_ = (unsigned)(pan_off[p + 1] - pan_off[p]);
+      const unsigned nt_ = (unsigned)(trail_off[p + 1] - trail_off[p]);
+      hipLaunchKernelGGL(bqr_panel, dim3(np_), dim3(256), 0, st, dWv, dTf, dqj, dpan + pan_off[p], p);
+      if (nt_)
+        hipLaunchKernelGGL(bqr_trail, dim3(nt_), dim3(256), 0, st, dWv, dTf, dqj,
+                           dtrail + trail_off[p], p);
+    }
+    hipLaunchKernelGGL(bqr_rvec, dim3(64, (unsigned)npre), dim3(256), 0, st, dWv, dqj);
+    TN_CHECK_LAUNCH();
   }
   // ---- 2. batched Jacobi
   hipLaunchKernelGGL(binit_identity, dim3((unsigned)ceil_div((int64_t)maxk * maxk, 256), njobs),
@@ -976,7 +1132,18 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
                      dsbase);
   hipLaunchKernelGGL(bsort_rank, dim3(njobs), dim3(1024), 0, st, dsig, dbj, dsbase, dperm, dS,
                      rel_threshold, chi_max, dkept);
-  hipLaunchKernelGGL(bout, dim3(256, njobs), dim3(256), 0, st, dWv, dZ, dbj, doj, dsbase, dperm, dS);
+  if (npre) {
+    // the left vectors of the preconditioned jobs: Q Z for the kept columns
+    hipLaunchKernelGGL(bqr_qz_init, dim3(256, (unsigned)npre), dim3(256), 0, st, dQZ, dZ, dqj,
+                       dperm, dsbase, dkept);
+    for (int p = maxpan - 1; p >= 0; --p) {
+      const unsigned na_ = (unsigned)(apply_off[p + 1] - apply_off[p]);
+      hipLaunchKernelGGL(bqr_apply, dim3(na_), dim3(256), 0, st, dWv, dTf, dqj,
+                         dapply + apply_off[p], dkept, p, dQZ);
+    }
+  }
+  hipLaunchKernelGGL(bout, dim3(256, njobs), dim3(256), 0, st, dWv, dZ, dbj, doj, dsbase, dperm, dS,
+                     dkept);
   TN_CHECK_LAUNCH();
   std::vector<int> done(njobs);
   if (kept_out)
@@ -1010,7 +1177,7 @@ extern "C" {
 size_t tn_batched_truncate_workspace(const tn_trunc_job* jobs, int njobs) {
   std::vector<int64_t> ms, ns;
   for (int j = 0; j < njobs && jobs; ++j) { ms.push_back(jobs[j].m); ns.push_back(jobs[j].n); }
-  return tn::batched_ws_bytes(ms, ns);
+  return tn::batched_ws_bytes(ms, ns, true);
 }
 
 int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshold, int chi_max,
@@ -1024,7 +1191,7 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
 size_t tn_batched_svd_workspace(const tn_svd_job* jobs, int njobs) {
   std::vector<int64_t> ms, ns;
   for (int j = 0; j < njobs && jobs; ++j) { ms.push_back(jobs[j].m); ns.push_back(jobs[j].n); }
-  return tn::batched_ws_bytes(ms, ns);
+  return tn::batched_ws_bytes(ms, ns, false);
 }
 
 int tn_batched_svd(const tn_svd_job* jobs, int njobs, int max_sweeps, int* sweeps_out, void* ws,
