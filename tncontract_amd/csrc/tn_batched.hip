@@ -23,12 +23,13 @@
 //    rank-deficient product lie below the keep threshold whatever their
 //    orthogonality, and orthogonalising that cluster was most of the sweeps
 //    of such jobs.
-//    Truncation jobs with 32 <= k and L <= 1024 are preconditioned by a
-//    Householder QR (tn_bqr.h, one workgroup per job): the Jacobi runs on
-//    the k rows of R (vectors of length k) and the left vectors are Q Z.
-//    On the graded, rank-deficient products of the truncation workload this
-//    takes ~8-11 sweeps where the columns of the product take ~20-40
-//    (simulated on the workload's spectra).
+//    Jobs with 32 < k and L <= 1024 are preconditioned by a
+//    Householder QR (tn_bqr.h: per panel, every job's panel in one launch
+//    and every job's trailing update in another): the Jacobi runs on the k
+//    rows of R (vectors of length k) and the left vectors are Q Z.  On the
+//    graded, rank-deficient products of the truncation workload (1024 jobs)
+//    this takes at most 12 sweeps where the columns of the products took up
+//    to 41.
 //  * per-job sort, rank selection (sigma/sigma_0 > threshold, <= chi) and
 //    output of U, S, V^T.
 #include "tn_bqr.h"
@@ -543,19 +544,20 @@ struct OJob {
 // (columns when m >= n, rows otherwise).
 struct AJob {
   const double* A;
+  int64_t x;  // offset of the job's vector matrix (L x k) in the vector buffer
 };
 
-__global__ void bgather_f64(double* __restrict__ Wv, const BJob* __restrict__ jobs,
-                            const AJob* __restrict__ ajobs, const OJob* __restrict__ outs) {
+__global__ void bgather_f64(double* __restrict__ Wv, const AJob* __restrict__ ajobs,
+                            const OJob* __restrict__ outs) {
   const int j = blockIdx.y;
-  const BJob J = jobs[j];
   const OJob O = outs[j];
   const double* A = ajobs[j].A;
-  const int64_t tot = (int64_t)J.L * J.k;
+  const int64_t L = O.tall ? O.m : O.n, k = O.tall ? O.n : O.m;
+  const int64_t tot = L * k;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = e / J.L, i = e - v * J.L;
-    Wv[J.wv + e] = O.tall ? A[i * O.n + v] : A[v * O.n + i];
+    const int64_t v = e / L, i = e - v * L;
+    Wv[ajobs[j].x + e] = O.tall ? A[i * O.n + v] : A[v * O.n + i];
   }
 }
 
@@ -671,11 +673,9 @@ static double time_grouped(int FR, const std::vector<int>& ids, const std::vecto
 }
 
 
-// QR preconditioning (tn_bqr.h) for the truncation jobs whose rows fit a
-// workgroup's registers and that have more than one block pair
-static bool qr_precond(bool trunc, int64_t L, int64_t k) {
-  return trunc && k > BJ2 && L <= kQrRows;
-}
+// QR preconditioning (tn_bqr.h) for the jobs whose rows fit a workgroup's
+// registers and that have more than one block pair
+static bool qr_precond(int64_t L, int64_t k) { return k > BJ2 && L <= kQrRows; }
 
 // Work items of a preconditioned job with k columns: one panel item per
 // panel; trailing-update groups of 64 columns after each panel; Q-apply
@@ -687,15 +687,14 @@ static void qr_item_counts(int64_t k, size_t* npan, size_t* ntrail, size_t* napp
   *napply += (size_t)(np * ceil_div(k, 64));
 }
 
-static size_t batched_ws_bytes(const std::vector<int64_t>& ms, const std::vector<int64_t>& ns,
-                               bool trunc) {
+static size_t batched_ws_bytes(const std::vector<int64_t>& ms, const std::vector<int64_t>& ns) {
   const int njobs = (int)ms.size();
   if (njobs <= 0) return 256;
   size_t wv = 0, z = 0, gi = 0, ei = 0, ai = 0, sv = 0, qz = 0, tf = 0, npre = 0;
   size_t npan = 0, ntrail = 0, napply = 0;
   for (int j = 0; j < njobs; ++j) {
     int64_t k = std::min(ms[j], ns[j]), L = std::max(ms[j], ns[j]);
-    const bool pre = qr_precond(trunc, L, k);
+    const bool pre = qr_precond(L, k);
     const int64_t Lj = pre ? k : L;  // Jacobi vector length
     int nblk = (int)ceil_div(std::max<int64_t>(k, 1), BJB);
     nblk += nblk % 2;
@@ -756,8 +755,7 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
     ms[j] = tj ? tj[j].m : sj[j].m;
     ns[j] = tj ? tj[j].n : sj[j].n;
   }
-  TN_REQUIRE(ws && ws_bytes >= batched_ws_bytes(ms, ns, tj != nullptr), "%s: workspace too small",
-             who);
+  TN_REQUIRE(ws && ws_bytes >= batched_ws_bytes(ms, ns), "%s: workspace too small", who);
   if (max_sweeps <= 0) max_sweeps = 40;
   hipStream_t st = as_stream(stream);
   // ---- host tables
@@ -786,11 +784,12 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
       TN_REQUIRE(J.m > 0 && J.n > 0 && sj[j].A && J.U && J.S && J.Vt, "%s: job %d is invalid",
                  who, j);
       aj[j].A = (const double*)sj[j].A;
+      aj[j].x = wv;
     }
     TN_REQUIRE(J.m <= (1 << 20) && J.n <= (1 << 20) && J.k <= (1 << 20), "%s: job %d too large",
                who, j);
     const int64_t k = std::min(J.m, J.n), L = std::max(J.m, J.n);
-    const bool pre = qr_precond(tj != nullptr, L, k);
+    const bool pre = qr_precond(L, k);
     const int64_t Lj = pre ? k : L;  // Jacobi vector length
     BJob b{};
     xoff[j] = wv;
@@ -949,9 +948,9 @@ static int batched_driver(const tn_trunc_job* tj, const tn_svd_job* sj, int njob
   //         contraction with the tile shape per size bucket (autotuned once)
   if (sj) {
     int64_t maxe = 1;
-    for (int j = 0; j < njobs; ++j) maxe = std::max<int64_t>(maxe, (int64_t)bj[j].L * bj[j].k);
+    for (int j = 0; j < njobs; ++j) maxe = std::max<int64_t>(maxe, ms[j] * ns[j]);
     hipLaunchKernelGGL(bgather_f64, dim3((unsigned)std::min<int64_t>(ceil_div(maxe, 256), 4096), njobs),
-                       dim3(256), 0, st, dWv, dbj, daj, doj);
+                       dim3(256), 0, st, dWv, daj, doj);
     TN_CHECK_LAUNCH();
   } else {
     std::vector<int> byb[4];
@@ -1177,7 +1176,7 @@ extern "C" {
 size_t tn_batched_truncate_workspace(const tn_trunc_job* jobs, int njobs) {
   std::vector<int64_t> ms, ns;
   for (int j = 0; j < njobs && jobs; ++j) { ms.push_back(jobs[j].m); ns.push_back(jobs[j].n); }
-  return tn::batched_ws_bytes(ms, ns, true);
+  return tn::batched_ws_bytes(ms, ns);
 }
 
 int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshold, int chi_max,
@@ -1191,7 +1190,7 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
 size_t tn_batched_svd_workspace(const tn_svd_job* jobs, int njobs) {
   std::vector<int64_t> ms, ns;
   for (int j = 0; j < njobs && jobs; ++j) { ms.push_back(jobs[j].m); ns.push_back(jobs[j].n); }
-  return tn::batched_ws_bytes(ms, ns, false);
+  return tn::batched_ws_bytes(ms, ns);
 }
 
 int tn_batched_svd(const tn_svd_job* jobs, int njobs, int max_sweeps, int* sweeps_out, void* ws,
