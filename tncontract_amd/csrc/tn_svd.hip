@@ -221,35 +221,51 @@ __global__ void __launch_bounds__(256) vec_norms(const double* __restrict__ Wv, 
 // NaN ranks as -inf so that the ranks stay a permutation of 0..k-1.
 __device__ __forceinline__ double rank_key(double x) { return x == x ? x : -INFINITY; }
 
-// One thread per j (grid ceil(k / 256) x 256); the keys are compared from
-// LDS tiles of 2048 (every thread reads the same key: a broadcast).
+// 256 elements j per block of 1024 threads: the four lanes l, l + 16,
+// l + 32, l + 48 of a wave count over every fourth key of an LDS tile of
+// 2048 (interleaved, so each reads from LDS in step with the others) and
+// their counts are summed by shuffles; the comparisons are unrolled by
+// eight so that the LDS reads of a group are in flight together.
 constexpr int RANK_TILE = 2048;
-__global__ void __launch_bounds__(256) rank_sort(const double* __restrict__ sig, int64_t k,
-                                                 int* __restrict__ perm, double* __restrict__ S) {
+__global__ void __launch_bounds__(1024) rank_sort(const double* __restrict__ sig, int64_t k,
+                                                  int* __restrict__ perm, double* __restrict__ S) {
   __shared__ double tile[RANK_TILE];
-  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4;
+  const int64_t j = (int64_t)blockIdx.x * 256 + wave * 16 + (lane & 15);
   const double sj = j < k ? rank_key(sig[j]) : 0.0;
-  int64_t rank = 0;
+  int cnt = 0;
   for (int64_t t0 = 0; t0 < k; t0 += RANK_TILE) {
     const int tn = (int)min<int64_t>(RANK_TILE, k - t0);
     __syncthreads();
-    for (int i = threadIdx.x; i < tn; i += 256) tile[i] = rank_key(sig[t0 + i]);
+    for (int i = threadIdx.x; i < tn; i += 1024) tile[i] = rank_key(sig[t0 + i]);
     __syncthreads();
-    int cnt = 0;
-    for (int i = 0; i < tn; ++i) {
-      const double si = tile[i];
-      cnt += (si > sj || (si == sj && t0 + i < j)) ? 1 : 0;
+    // this lane's keys: i = q + 4 m
+    const int64_t jr = j - t0;  // key i precedes j on ties when i < jr
+    int i = q;
+    for (; i + 28 < tn; i += 32) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = tile[i + 4 * u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        cnt += (v[u] > sj || (v[u] == sj && i + 4 * u < jr)) ? 1 : 0;
     }
-    rank += cnt;
+    for (; i < tn; i += 4) {
+      const double si = tile[i];
+      cnt += (si > sj || (si == sj && i < jr)) ? 1 : 0;
+    }
   }
-  if (j < k) {
-    perm[rank] = (int)j;
-    S[rank] = sig[j];
+  cnt += __shfl_xor(cnt, 16, 64);
+  cnt += __shfl_xor(cnt, 32, 64);
+  if (j < k && q == 0) {
+    perm[cnt] = (int)j;
+    S[cnt] = sig[j];
   }
 }
 
 static void launch_rank_sort(const double* sig, int64_t k, int* perm, double* S, hipStream_t st) {
-  hipLaunchKernelGGL(rank_sort, dim3((unsigned)ceil_div(std::max<int64_t>(k, 1), 256)), dim3(256),
+  hipLaunchKernelGGL(rank_sort, dim3((unsigned)ceil_div(std::max<int64_t>(k, 1), 256)), dim3(1024),
                      0, st, sig, k, perm, S);
 }
 
