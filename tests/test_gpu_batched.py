@@ -72,3 +72,30 @@ def test_config3_full_size(tn):
     for j in order[::len(order) // 48][:48]:
         Lm, Rm, _ = jobs[j]
         check_job(Lm, Rm, U[j], S[j], Vt[j], int(kept[j]), 1e-8, 0)
+
+
+def test_batched_svd_qr_preconditioned_edges(tn):
+    """Jobs at the edges of the Householder QR preconditioning of the batched
+    Jacobi (k = min(m, n) > 32 and max(m, n) <= 1024 are preconditioned): the
+    smallest preconditioned k (33), exactly 1024 rows, a square 1024 x 1024
+    job, a 1025-row job beside them (left unpreconditioned), a rank-10
+    product and an all-zero matrix, in one batch against LAPACK.  Exactly
+    zero singular values get zero vectors (the library's convention), so the
+    orthonormality is checked on the nonzero part."""
+    from tncontract_amd.batched import batched_svd
+    rng = np.random.default_rng(33)
+    shapes = [(33, 40), (40, 33), (1024, 200), (200, 1024), (1024, 1024), (1025, 64), (64, 1025)]
+    mats = [rng.standard_normal(s) for s in shapes]
+    mats.append(rng.standard_normal((300, 10)) @ rng.standard_normal((10, 100)))
+    mats.append(np.zeros((64, 80)))
+    U, S, Vt, sweeps = batched_svd([torch.from_numpy(a).cuda() for a in mats])
+    for a, u, s, vt in zip(mats, U, S, Vt):
+        u, s, vt = u.cpu().numpy(), s.cpu().numpy(), vt.cpu().numpy()
+        ref = np.linalg.svd(a, compute_uv=False)
+        scale = max(ref[0], 1.0)
+        np.testing.assert_allclose(s, ref, rtol=1e-11, atol=1e-12 * scale)
+        np.testing.assert_allclose((u * s) @ vt, a, atol=1e-11 * scale)
+        r = int(np.count_nonzero(ref > 1e-10 * scale))
+        np.testing.assert_allclose(u[:, :r].T @ u[:, :r], np.eye(r), atol=1e-11)
+        np.testing.assert_allclose(vt[:r] @ vt[:r].T, np.eye(r), atol=1e-11)
+    assert sweeps.max() <= 20, sweeps

@@ -16,8 +16,8 @@
 //     the panel's other columns and the new column of T from the same dots);
 //   * bqr_trail, one workgroup per 64 trailing columns of a job: C -= V T^T
 //     V^T C by v_mfma_f64_16x16x4, each wave a 16-column tile, V read back
-//     from the stored panel.  Few registers, so many waves per CU hide the
-//     loads' latency.
+//     from the stored panel through LDS.  Few registers, so many waves per
+//     CU hide the loads' latency.
 #pragma once
 #include "tn_common.h"
 
@@ -58,51 +58,69 @@ __device__ __forceinline__ void qr_block_sum(double (&v)[N], double (*red)[QNB])
   __syncthreads();
 }
 
-// C[p0:L, c0:c1) -= V op(T) V^T C[p0:L, c0:c1) (column-major C, ld = ldc),
-// V the nb reflectors of the panel at column p0 of X (ld = L; unit diagonal,
-// zero above it), op(T) = T^T if trans (applying Q^T) else T.  Wave w takes
-// the 16-column tiles w, w + 4, ...  MFMA operand layouts (16x16x4 f64):
-// A[m][k] in lane m + 16 k, B[k][n] in lane n + 16 k, D[k + 4 e][n] in
-// element e of lane n + 16 k.
+// C[p0:L, c0:c1) -= V op(T) V^T C[p0:L, c0:c1) for c1 - c0 <= 64 (column-
+// major C, ld = ldc; called by the whole block: wave w takes the 16-column
+// tile at c0 + 16 w), V the nb reflectors of the panel at column p0 of X
+// (ld = L; unit diagonal, zero above it), op(T) = T^T if trans (applying
+// Q^T) else T.  V is staged through LDS (Vs) 64 rows at a time with
+// coalesced loads shared by the four waves; each wave's C loads of a chunk
+// are issued together.  MFMA operand layouts (16x16x4 f64): A[m][k] in lane
+// m + 16 k, B[k][n] in lane n + 16 k, D[k + 4 e][n] in element e of lane
+// n + 16 k.
 __device__ __forceinline__ void qr_block_reflect(const double* __restrict__ X, int L, int p0,
                                                  int nb, const double (*Ts)[QNB + 1], bool trans,
                                                  double* __restrict__ C, int64_t ldc, int c0,
-                                                 int c1) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+                                                 int c1, double (*Vs)[QNB + 1]) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
-  auto vent = [&](int row, int c) -> double {  // V[row][c]
-    const int dr = p0 + c;
-    if (c >= nb || row >= L || row < dr) return 0.0;
-    return row == dr ? 1.0 : X[(int64_t)dr * L + row];
-  };
-  for (int t0 = c0 + 16 * wave; t0 < c1; t0 += 64) {
-    const int cc = t0 + lr;
-    const bool cok = cc < c1;
-    double* Cc = C + (int64_t)(cok ? cc : c0) * ldc;
-    // Y = V^T C_tile (16 x 16), summed over the rows p0 .. L-1
-    f64x4q y = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int r = p0; r < L; r += 4) {
-      const int row = r + lk;
-      const double a = vent(row, lr);
-      const double b = (cok && row < L) ? Cc[row] : 0.0;
-      y = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, y, 0, 0, 0);
-    }
-    // Y2 = op(T) Y: element u of y holds Y[lk + 4u][lr], the B operand of
-    // the k-step whose summation indices are lk + 4u
-    f64x4q y2 = {0.0, 0.0, 0.0, 0.0};
+  const int cc = c0 + 16 * wave + lr;
+  const bool cok = cc < c1;
+  double* Cc = C + (int64_t)(cok ? cc : c0) * ldc;
+  // V rows r0 .. r0 + 63 into Vs (zero past L, above the diagonal and past nb)
+  auto stage = [&](int r0) {
+    __syncthreads();
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int kk = lk + 4 * u;
-      const double a = trans ? Ts[kk][lr] : Ts[lr][kk];
-      y2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, y[u], y2, 0, 0, 0);
+      const int e = tid + 256 * u, i = e & 63, c = e >> 6, row = r0 + i, dr = p0 + c;
+      double v = 0.0;
+      if (c < nb && row < L && row >= dr) v = row == dr ? 1.0 : X[(int64_t)dr * L + row];
+      Vs[i][c] = v;
     }
-    // C_tile -= V Y2, 16 rows at a time
-    for (int R0 = p0; R0 < L; R0 += 16) {
+    __syncthreads();
+  };
+  // Y = V^T C_tile (16 x 16), summed over the rows p0 .. L-1
+  f64x4q y = {0.0, 0.0, 0.0, 0.0};
+  for (int r0 = p0; r0 < L; r0 += 64) {
+    stage(r0);
+    double b[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int row = r0 + 4 * i + lk;
+      b[i] = (cok && row < L) ? Cc[row] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      y = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[4 * i + lk][lr], b[i], y, 0, 0, 0);
+  }
+  // Y2 = op(T) Y: element u of y holds Y[lk + 4u][lr], the B operand of
+  // the k-step whose summation indices are lk + 4u
+  f64x4q y2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int kk = lk + 4 * u;
+    const double a = trans ? Ts[kk][lr] : Ts[lr][kk];
+    y2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, y[u], y2, 0, 0, 0);
+  }
+  // C_tile -= V Y2, 16 rows at a time
+  for (int r0 = p0; r0 < L; r0 += 64) {
+    stage(r0);
+#pragma unroll
+    for (int s16 = 0; s16 < 4; ++s16) {
+      const int R0 = r0 + 16 * s16;
       f64x4q d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        d = __builtin_amdgcn_mfma_f64_16x16x4f64(vent(R0 + lr, lk + 4 * u), y2[u], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[16 * s16 + lr][lk + 4 * u], y2[u], d, 0, 0, 0);
       if (cok) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -231,15 +249,16 @@ __global__ void __launch_bounds__(256) bqr_trail(double* __restrict__ Wv,
                                                  const QJob* __restrict__ qjobs,
                                                  const QItem* __restrict__ items, int p) {
   __shared__ double Ts[QNB][QNB + 1];
+  __shared__ double Vs[64][QNB + 1];
   const QItem it = items[blockIdx.x];
   const QJob Q = qjobs[it.q];
   const int tid = threadIdx.x;
   const int p0 = p * QNB, nb = min(QNB, Q.k - p0);
   const int c0 = p0 + nb + 64 * it.g, c1 = min(Q.k, c0 + 64);
   Ts[tid >> 4][tid & 15] = Tf[Q.tf + (int64_t)p * (QNB * QNB) + tid];
-  __syncthreads();
+  // (the first staging barrier orders Ts before its use)
   double* X = Wv + Q.x;
-  qr_block_reflect(X, Q.L, p0, nb, Ts, true, X, Q.L, c0, c1);
+  qr_block_reflect(X, Q.L, p0, nb, Ts, true, X, Q.L, c0, c1, Vs);
 }
 
 // The rows of R as vectors: Rv[c * k + i] = R[c][i] (i >= c), 0 below.
@@ -281,16 +300,16 @@ __global__ void __launch_bounds__(256) bqr_apply(const double* __restrict__ Wv,
                                                  const int* __restrict__ ncols, int p,
                                                  double* __restrict__ QZ) {
   __shared__ double Ts[QNB][QNB + 1];
+  __shared__ double Vs[64][QNB + 1];
   const QItem it = items[blockIdx.x];
   const QJob Q = qjobs[it.q];
   const int nc = ncols[Q.job];
   const int c0 = 64 * it.g, c1 = min(nc, c0 + 64);
-  if (c0 >= c1) return;
+  if (c0 >= c1) return;  // block-uniform
   const int tid = threadIdx.x;
   const int p0 = p * QNB, nb = min(QNB, Q.k - p0);
   Ts[tid >> 4][tid & 15] = Tf[Q.tf + (int64_t)p * (QNB * QNB) + tid];
-  __syncthreads();
-  qr_block_reflect(Wv + Q.x, Q.L, p0, nb, Ts, false, QZ + Q.qz, Q.L, c0, c1);
+  qr_block_reflect(Wv + Q.x, Q.L, p0, nb, Ts, false, QZ + Q.qz, Q.L, c0, c1, Vs);
 }
 
 }  // namespace tn
