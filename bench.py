@@ -61,7 +61,7 @@ def parse():
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--eig-tol", type=float, default=None,
                    help="Jacobi tolerance of the fast path's eigensolver (default: the "
-                        "structured sweep's own, 1e-10 for mixed precision)")
+                        "structured sweep's own, 1e-8 for mixed precision)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     p.add_argument("--no-extra", action="store_true",
                    help="headline only: skip the config 3 and config 4 runs reported in its line")
