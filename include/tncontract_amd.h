@@ -223,12 +223,15 @@ int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes* k_cols, i
  * relative threshold -- here for many independent jobs of different sizes in
  * shared launches (grouped GEMM of the FP32 factors on FP64 MFMA with FP64
  * accumulation and autotuned tile shapes, then a batched one-sided block
- * Jacobi with FP64 rotations).
+ * Jacobi with FP64 rotations; a job with 32 < min(m, n) and max(m, n) <=
+ * 1024 is first factored C_j = Q R by a batched Householder QR, the Jacobi
+ * runs on the rows of R and the left vectors are Q Z).
  * C_j = L_j R_j (m x n); C_j = U S V^T (thin, sorted descending).  kept_out[j]
  * (host) = #{sigma/sigma_0 > rel_threshold}, capped at chi_max (>0): C_j is
  * FP64-accurate, so this is the rank of the exact product.  The first kept
  * columns of U / rows of Vt are the truncated factors.  The Jacobi stops at
- * orthogonality L eps (L the vector length); vectors whose squared norm is
+ * orthogonality max(1e-10, L eps) (L the Jacobi vector length: max(m, n),
+ * or min(m, n) when QR-preconditioned); vectors whose squared norm is
  * below (rel_threshold / 2k)^2 ||C_j||_F^2 are not rotated against each
  * other (their span lies below the threshold), so the singular values and
  * vectors past the kept ones are not those of an SVD. */
@@ -247,9 +250,10 @@ int tn_batched_truncate(const tn_trunc_job* jobs, int njobs, double rel_threshol
 
 /* Many independent FP64 thin SVDs in shared launches: np.linalg.svd
  * (tensor.py:915) of each job's A (m x n, FP64, row-major), A = U S Vt with S
- * descending; the same batched one-sided block Jacobi (FP64 rotations),
- * orthogonality to L eps (L = max(m, n)).  All min(m, n) singular triplets
- * are returned; the caller truncates.  No counterpart as a batch in the
+ * descending; the same batched one-sided block Jacobi (FP64 rotations,
+ * QR-preconditioned as above), orthogonality to L eps (L the Jacobi vector
+ * length).  All min(m, n) singular triplets are returned, an exactly zero
+ * singular value with zero vectors; the caller truncates.  No counterpart as a batch in the
  * reference: it serves a layer of TEBD gates on disjoint bonds
  * (onedim_core.py:1100-1186), whose SVDs are independent. */
 typedef struct tn_svd_job {
