@@ -120,6 +120,43 @@ def test_svd_graded_spectrum(L):
     np.testing.assert_allclose(sg, s, rtol=1e-9, atol=1e-14)
 
 
+def test_eigh_psd_ties_across_sort_tiles(L):
+    """The device rank sort with repeated keys and more keys than one LDS
+    tile (2304 > 2048): the PSD eigensolver of a diagonal matrix whose 2304
+    entries take 24 distinct values sorts them twice (the static pivot and
+    the final order).  Eigenvalues must be the sorted diagonal and U a
+    signed permutation matching them."""
+    from tncontract_amd import tensor as T
+    n = 2304
+    rng = np.random.default_rng(5)
+    diag = rng.permutation(np.repeat(np.arange(1.0, 25.0), n // 24))
+    u, lam = T._eigh_psd_device(dev(np.diag(diag)), 1)
+    u, lam = u.cpu().numpy(), lam.cpu().numpy()
+    np.testing.assert_allclose(lam, np.sort(diag)[::-1], rtol=1e-14, atol=0)
+    au = np.abs(u)
+    np.testing.assert_allclose(au.max(axis=0), np.ones(n), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(au.sum(axis=0), np.ones(n), rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(diag[np.argmax(au, axis=0)], np.sort(diag)[::-1])
+    assert len(set(np.argmax(au, axis=0).tolist())) == n  # a permutation
+
+
+def test_svd_repeated_singular_values(L):
+    """Equal singular values (a scaled orthogonal block beside a graded one)
+    through the Jacobi's per-sweep norm sort and the final rank sort."""
+    from tncontract_amd import tensor as T
+    rng = np.random.default_rng(6)
+    m, n = 700, 520
+    uu, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    vv, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.concatenate([np.full(300, 2.0), np.exp(-np.arange(220) / 40.0)])
+    a = (uu * s) @ vv.T
+    u, sg, vt = T._svd_device(dev(a), 1)
+    u, sg, vt = u.cpu().numpy(), sg.cpu().numpy(), vt.cpu().numpy()
+    np.testing.assert_allclose(sg, np.sort(s)[::-1], rtol=1e-12, atol=1e-13 * 2.0)
+    np.testing.assert_allclose((u * sg) @ vt, a, atol=1e-12 * 2.0 * np.sqrt(n))
+    np.testing.assert_allclose(u.T @ u, np.eye(n), atol=1e-12 * np.sqrt(n))
+
+
 def test_sumsq_and_scale(L):
     rng = np.random.default_rng(9)
     a = rng.standard_normal(1000003)
