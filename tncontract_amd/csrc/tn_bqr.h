@@ -47,7 +47,7 @@ template <int N>
 __device__ __forceinline__ void qr_block_sum(double (&v)[N], double (*red)[QNB]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int c = 0; c < N; ++c) v[c] = wave_sum(v[c]);
+  for (int c = 0; c < N; ++c) v[c] = wave_sum_dpp(v[c]);
   if (lane == 0) {
 #pragma unroll
     for (int c = 0; c < N; ++c) red[wave][c] = v[c];

@@ -110,6 +110,33 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// One DPP step of a wave reduction on an FP64 value: the 32-bit halves move
+// with the same control; lanes of rows outside ROWMASK receive 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROWMASK, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the wave by DPP (VALU moves, no LDS traffic): quad butterflies,
+// half-row and row mirrors give every lane its 16-lane row sum, row
+// broadcasts 15 and 31 carry rows 0..2 into lane 63, whose value every lane
+// then reads.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_f64<0x140, 0xF>(v);  // row_mirror
+  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)u, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll

@@ -35,7 +35,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], int nv, double (*lds)
   par ^= 1;
 #pragma unroll
   for (int q = 0; q < NV; ++q)
-    if (q < nv) v[q] = wave_sum(v[q]);
+    if (q < nv) v[q] = wave_sum_dpp(v[q]);
   if (lane == 0) {
 #pragma unroll
     for (int q = 0; q < NV; ++q)
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(256) qr_panel_dots(const double* __restrict__ 
 #pragma unroll
   for (int a = 0; a < W; ++a)
 #pragma unroll
-    for (int j = 0; j < QCOL; ++j) y[a][j] = wave_sum(y[a][j]);
+    for (int j = 0; j < QCOL; ++j) y[a][j] = wave_sum_dpp(y[a][j]);
   if ((tid & 63) == 0) {
 #pragma unroll
     for (int a = 0; a < W; ++a)
