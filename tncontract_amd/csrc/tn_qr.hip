@@ -23,13 +23,17 @@ constexpr int PT = 512;  // panel threads (2 waves per SIMD: 256 VGPRs per threa
 constexpr int NWAVE = PT / 64;
 
 // Block-wide sum of nv (<= NV) values; every thread receives the totals.
-// One barrier per call: each wave's partials go to one of two LDS buffers
-// (alternating with `par`) and every thread sums the NWAVE partials itself
-// in wave order.  A buffer is written again only two calls later, after the
-// intervening call's barrier, which every thread reaches after its reads.
+// One barrier per call: each wave's partials (DPP-reduced) go to one of two
+// LDS buffers (alternating with `par`); then lane l of every wave reads the
+// one partial (wave l / NV, value l % NV), three xor-shuffles sum each
+// value over the waves, and every lane takes the NV totals from lanes
+// 0 .. NV-1 (bit-identical in all threads).  A buffer is written again only
+// two calls later, after the intervening call's barrier, which every thread
+// reaches after its read.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], int nv, double (*lds)[NWAVE * NV],
                                           int& par) {
+  static_assert(NWAVE * NV <= 64, "one partial per lane");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double* buf = lds[par];
   par ^= 1;
@@ -42,13 +46,16 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], int nv, double (*lds)
       if (q < nv) buf[wave * NV + q] = v[q];
   }
   __syncthreads();
+  double p = lane < NWAVE * NV ? buf[lane] : 0.0;
+#pragma unroll
+  for (int sh = NV; sh < NWAVE * NV; sh <<= 1) p += __shfl_xor(p, sh, 64);
+  const long long u = __double_as_longlong(p);
 #pragma unroll
   for (int q = 0; q < NV; ++q)
     if (q < nv) {
-      double acc = 0.0;
-#pragma unroll
-      for (int w = 0; w < NWAVE; ++w) acc += buf[w * NV + q];
-      v[q] = acc;
+      const int lo = __builtin_amdgcn_readlane((int)u, q);
+      const int hi = __builtin_amdgcn_readlane((int)(u >> 32), q);
+      v[q] = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
     }
 }
 
