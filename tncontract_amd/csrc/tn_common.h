@@ -137,6 +137,23 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// Maximum over the wave of an FP32 value by DPP, as wave_sum_dpp (exact in
+// any order, so bit-identical to a shuffle reduction).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f32(float v, float fill) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), CTRL,
+                                                    ROWMASK, 0xF, false));
+}
+__device__ __forceinline__ float wave_max_dpp(float v) {
+  v = fmaxf(v, dpp_f32<0xB1, 0xF>(v, v));
+  v = fmaxf(v, dpp_f32<0x4E, 0xF>(v, v));
+  v = fmaxf(v, dpp_f32<0x141, 0xF>(v, v));
+  v = fmaxf(v, dpp_f32<0x140, 0xF>(v, v));
+  v = fmaxf(v, dpp_f32<0x142, 0xA>(v, v));
+  v = fmaxf(v, dpp_f32<0x143, 0xC>(v, v));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll

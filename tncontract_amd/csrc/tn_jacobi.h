@@ -95,7 +95,7 @@ __device__ inline double eig_off(EigLds& sm, double tol, double neg2 = 0.0) {
       if (rot) off = fmaxf(off, fmaxf(v, above));
     }
   }
-  off = wave_max(off);
+  off = wave_max_dpp(off);
   if ((tid & 63) == 0) sm.red[tid >> 6] = off;
   lds_barrier();
   off = fmaxf(fmaxf(sm.red[0], sm.red[1]), fmaxf(sm.red[2], sm.red[3]));
