@@ -1,6 +1,6 @@
 // Batched Householder QR of many independent column-major L x k matrices
-// (k <= L <= kQrRows), one workgroup per matrix, and the application of their
-// Q factors: the preconditioner of the batched one-sided Jacobi
+// (k <= L <= kQrRows), panel by panel over all of them at once, and the
+// application of their Q factors: the preconditioner of the batched one-sided Jacobi
 // (tn_batched.hip).  One-sided Jacobi on the rows of R instead of the
 // columns of X = Q R converges in far fewer sweeps on graded and
 // rank-deficient matrices (Drmac & Veselic, "New fast and accurate Jacobi
@@ -42,9 +42,13 @@ struct QJob {
   int32_t pad;
 };
 
-// v[0..N) summed over the block's 256 threads; every thread gets the sums.
+// v[0..N) summed over the block's 256 threads; every thread gets the sums
+// (bit-identical in all of them): each wave's values by DPP, the four waves'
+// partials one per lane from LDS, summed by two shuffles and read back by
+// readlane.
 template <int N>
 __device__ __forceinline__ void qr_block_sum(double (&v)[N], double (*red)[QNB]) {
+  static_assert(4 * N <= 64, "one partial per lane");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < N; ++c) v[c] = wave_sum_dpp(v[c]);
@@ -53,8 +57,16 @@ __device__ __forceinline__ void qr_block_sum(double (&v)[N], double (*red)[QNB])
     for (int c = 0; c < N; ++c) red[wave][c] = v[c];
   }
   __syncthreads();
+  double p = lane < 4 * N ? red[lane / N][lane % N] : 0.0;
 #pragma unroll
-  for (int c = 0; c < N; ++c) v[c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+  for (int sh = N; sh < 4 * N; sh <<= 1) p += __shfl_xor(p, sh, 64);
+  const long long u = __double_as_longlong(p);
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    const int lo = __builtin_amdgcn_readlane((int)u, c);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), c);
+    v[c] = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+  }
   __syncthreads();
 }
 
