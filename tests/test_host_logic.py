@@ -1,5 +1,5 @@
 """Host-side logic of the GPU path that needs no GPU: the structured sweep's
-bond-rank rule and its resolution floor."""
+bond-rank rule and its resolution floor, and bench.py's one-GPU contract."""
 import numpy as np
 import pytest
 
@@ -74,3 +74,17 @@ def test_right_sweep_bond_numbering():
     # reversed position i splits original sites N-2-i | N-1-i, i.e. bond N-1-i
     from tncontract_amd.learned import _bond_of
     assert [_bond_of(i, 6) for i in range(5)] == [5, 4, 3, 2, 1]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says
+    so, before any GPU work."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300, cwd=root)
+    assert r.returncode == 2
+    assert "--gpus 2 is not supported" in r.stderr
+    assert r.stdout == ""
