@@ -1,9 +1,10 @@
 """Learned-rank truncation: a small MLP predicts, from the previous sweep's
 spectrum at each bond, the rank a sketch of that bond needs to certify the
 keep rule, so a randomized range-finder SVD
-(Halko, Martinsson & Tropp 2011: Gaussian sketch, r + 10 columns, 2 power
-iterations with Householder re-orthogonalisation) can replace the full
-Jacobi SVD in the truncating sweep of svd_compress.  The sketch is accepted
+(Halko, Martinsson & Tropp 2011: Gaussian sketch of r + p columns, optional
+power iterations with Householder re-orthogonalisation; the sweep uses
+p = 50 and none) can replace the full Jacobi SVD in the truncating sweep of
+svd_compress.  The sketch is accepted
 only when it certifies the reference's keep rule (sigma / sigma_0 >
 threshold): an upper bound on the spectral norm of the residual
 A - U S V^T, holding with probability at least 1 - 1e-10 (the posterior
@@ -259,8 +260,21 @@ class LearnedSweepStats:
         return sum(1 for p in self.path.values() if p == "fallback")
 
 
+# The sketch shape of the learned sweep: no power iteration and 50
+# oversampling columns.  Measured on config 4 (against the same sweep with
+# the full SVD at every bond): q = 2, p = 10: 0.87x, 2 fallbacks; q = 1,
+# p = 20: 1.01x, 1; q = 0, p = 30: 1.12x, 1; q = 0, p = 50: 1.20x, none;
+# p = 70 or 100 send more bonds to the full SVD (the sketch would exceed
+# half of min(m, n)): 1.12x, 1.06x.  The decaying bond spectra make the
+# oversampling, not the power iterations (two Householder QRs each), the
+# cheaper way to certify.
+SWEEP_OVERSAMPLE = 50
+SWEEP_POWER_ITERS = 0
+
+
 def learned_svd_compress(mps, chi, threshold, model, prev_spectra, stats=None, margin=16,
-                         scale=1.2, oversample=10, power_iters=2, seed=0):
+                         scale=1.2, oversample=SWEEP_OVERSAMPLE, power_iters=SWEEP_POWER_ITERS,
+                         seed=0):
     """svd_compress(chi, threshold) (onedim_core.py:463-484) of a copy of
     `mps` with each truncating SVD of the right sweep done by certified_svd
     at the rank `model` predicts from prev_spectra[bond] (sigma / sigma_0 of
