@@ -230,7 +230,7 @@ int tn_eigh_psd(const void* K, const tn_modes* k_rows, const tn_modes* k_cols, i
  * (host) = #{sigma/sigma_0 > rel_threshold}, capped at chi_max (>0): C_j is
  * FP64-accurate, so this is the rank of the exact product.  The first kept
  * columns of U / rows of Vt are the truncated factors.  The Jacobi stops at
- * orthogonality max(1e-10, L eps) (L the Jacobi vector length: max(m, n),
+ * orthogonality max(1e-8, L eps) (L the Jacobi vector length: max(m, n),
  * or min(m, n) when QR-preconditioned); vectors whose squared norm is
  * below (rel_threshold / 2k)^2 ||C_j||_F^2 are not rotated against each
  * other (their span lies below the threshold), so the singular values and
