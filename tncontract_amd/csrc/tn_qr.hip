@@ -362,37 +362,80 @@ __global__ void qr_build_v_kernel(const double* __restrict__ A, int64_t m, int64
 }
 
 // T (nb x nb, row-major) from G = V^T V and tau (LAPACK dlarft,
-// forward/columnwise): T[b][b] = tau_b, T[a][b] = -tau_b sum_{a<=c<b}
-// T[a][c] G[c][b].  One wavefront, lane a owns row a of T in registers:
-// row a only needs its own earlier entries and G, so the 64 dependent
-// columns run without any barrier (fully unrolled, static register indices).
-__global__ void __launch_bounds__(64) qr_larft_kernel(const double* __restrict__ G, int nb,
-                                                      const double* __restrict__ tau,
-                                                      double* __restrict__ T) {
+// forward/columnwise: H_1 ... H_nb = I - V T V^T), recursively: with the
+// reflectors split into blocks A | B,
+//   T = [[T_A, -T_A G_AB T_B], [0, T_B]].
+// Wave w builds the 16 x 16 diagonal block w by the column recurrence
+// T[a][b] = -tau_b sum_{a<=c<b} T[a][c] G[c][b] (lane a owns row a; only
+// 16 dependent columns), then all 256 threads form the off-diagonal blocks
+// of the two 32 x 32 halves and finally the 32 x 32 block between them,
+// each as two small products through LDS.  Indices >= nb carry tau = 0 and
+// G = 0, so their rows and columns of T are zero.
+__global__ void __launch_bounds__(256) qr_larft_kernel(const double* __restrict__ G, int nb,
+                                                       const double* __restrict__ tau,
+                                                       double* __restrict__ T) {
   __shared__ double Gs[QR_NB][QR_NB + 1];
+  __shared__ double Ts[QR_NB][QR_NB + 1];
+  __shared__ double Ms[QR_NB][QR_NB + 1];
   __shared__ double ts[QR_NB];
-  const int a = threadIdx.x;
-  for (int i = a; i < nb * nb; i += 64) Gs[i / nb][i % nb] = G[i];
-  if (a < nb) ts[a] = tau[a];
-  __syncthreads();
-  double Tr[QR_NB];
+  const int tid = threadIdx.x;
+  {
+    double g[QR_NB * QR_NB / 256];
 #pragma unroll
-  for (int b = 0; b < QR_NB; ++b) {
-    double v = 0.0;
-    if (b < nb) {
-      double acc = 0.0;
-#pragma unroll
-      for (int c = 0; c < b; ++c)
-        if (c >= a) acc = fma(Tr[c], Gs[c][b], acc);
-      v = a == b ? ts[b] : (a < b ? -ts[b] * acc : 0.0);
+    for (int k = 0; k < QR_NB * QR_NB / 256; ++k) {
+      const int i = tid + 256 * k, r = i / QR_NB, c = i % QR_NB;
+      g[k] = (r < nb && c < nb) ? G[r * nb + c] : 0.0;
     }
-    Tr[b] = v;
-  }
-  if (a < nb) {
 #pragma unroll
-    for (int b = 0; b < QR_NB; ++b)
-      if (b < nb) T[a * nb + b] = Tr[b];
+    for (int k = 0; k < QR_NB * QR_NB / 256; ++k) {
+      const int i = tid + 256 * k;
+      Gs[i / QR_NB][i % QR_NB] = g[k];
+      Ts[i / QR_NB][i % QR_NB] = 0.0;
+    }
   }
+  if (tid < QR_NB) ts[tid] = tid < nb ? tau[tid] : 0.0;
+  __syncthreads();
+  // diagonal 16 x 16 blocks: wave w, lane a < 16 owns row 16 w + a
+  {
+    const int w = tid >> 6, a = tid & 63, o = 16 * w;
+    if (a < 16) {
+      double Tr[16];
+#pragma unroll
+      for (int bb = 0; bb < 16; ++bb) {
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < bb; ++c)
+          if (c >= a) acc = fma(Tr[c], Gs[o + c][o + bb], acc);
+        Tr[bb] = a == bb ? ts[o + bb] : (a < bb ? -ts[o + bb] * acc : 0.0);
+      }
+#pragma unroll
+      for (int bb = 0; bb < 16; ++bb) Ts[o + a][o + bb] = Tr[bb];
+    }
+  }
+  __syncthreads();
+  // off-diagonal block of the half at h0 (size 2 hs, hs = 16 then 32):
+  // T[h0 + i][h0 + hs + j] = -sum_p T_A[i][p] (sum_q G_AB[p][q] T_B[q][j])
+  auto combine = [&](int hs, int nh) {
+    // M = G_AB T_B for nh halves at once: entries (half, p, j)
+    const int per = hs * hs;
+    for (int e = tid; e < nh * per; e += 256) {
+      const int hh = e / per, p = (e % per) / hs, j = e % hs, h0 = 2 * hs * hh;
+      double acc = 0.0;
+      for (int q = 0; q <= j; ++q) acc = fma(Gs[h0 + p][h0 + hs + q], Ts[h0 + hs + q][h0 + hs + j], acc);
+      Ms[h0 + p][h0 + hs + j] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < nh * per; e += 256) {
+      const int hh = e / per, i = (e % per) / hs, j = e % hs, h0 = 2 * hs * hh;
+      double acc = 0.0;
+      for (int p = i; p < hs; ++p) acc = fma(Ts[h0 + i][h0 + p], Ms[h0 + p][h0 + hs + j], acc);
+      Ts[h0 + i][h0 + hs + j] = -acc;
+    }
+    __syncthreads();
+  };
+  combine(16, 2);
+  combine(32, 1);
+  for (int e = tid; e < nb * nb; e += 256) T[e] = Ts[e / nb][e % nb];
 }
 
 __global__ void qr_init_from_kernel(const double* __restrict__ X, double* __restrict__ Q, int64_t m,
@@ -637,7 +680,7 @@ int qr_factor(double* W, int64_t m, int64_t n, const QrWs& ws, hipStream_t st) {
                            mk({nbo}, {mr}), ws.G, mk({nbo}, {nbo}), mk({nbo}, {1}), TN_F64,
                            TN_PREC_NATIVE, 1.0, 0.0, ws.part, ws.part_bytes, st));
     double* T = ws.Tall + (j0 / QR_NB) * QR_NB * QR_NB;
-    hipLaunchKernelGGL(qr_larft_kernel, dim3(1), dim3(64), 0, st, ws.G, nbo, ws.tau + j0, T);
+    hipLaunchKernelGGL(qr_larft_kernel, dim3(1), dim3(256), 0, st, ws.G, nbo, ws.tau + j0, T);
     TN_CHECK_LAUNCH();
     const int64_t nt = n - (j0 + nbo);
     if (nt > 0) {
