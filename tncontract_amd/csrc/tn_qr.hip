@@ -61,9 +61,11 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], int nv, double (*lds)
 
 // Factor columns [c0, c0+w) (w <= W) over rows [c0, m) of A (column-major,
 // ld), one workgroup, the panel in registers: thread t owns rows
-// c0 + t + PT*i.  Per column: one reduction for ||x_j|| (which also carries
-// the Gram entries v_a . v_{j-1}, a < j-1, of the compact-WY factor) and one
-// for the update dot products.
+// c0 + t + PT*i.  Per column one block reduction, carrying ||x_j|| below the
+// pivot, the Gram entries v_a . v_{j-1} (a < j-1) of the compact-WY factor
+// and the unscaled dot products of x_j with the later columns below the
+// pivot, from which the update's dots v . x_q follow without a second
+// reduction.
 template <int R, int W>
 __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, int64_t m,
                                                       int64_t ld, int64_t c0, int w,
@@ -72,6 +74,7 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
   __shared__ double lds[2][NWAVE * W];
   int par = 0;
   __shared__ double piv[W];       // pivot value of each column (one slot each)
+  __shared__ double prow[W][W];   // prow[j][q]: pivot row of column j, entry q > j (one row each)
   __shared__ double gsh[W][W];    // v_a . v_b, a < b
   __shared__ double taus[W];
   const int t = threadIdx.x;
@@ -87,7 +90,11 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
 #pragma unroll
   for (int j = 0; j < W; ++j) {
     if (j < w) {
-      // vals[0] = ||x_j below the pivot||^2; vals[1 + a] = v_a . v_{j-1}, a < j-1
+      // One reduction per column.  vals[0] = ||x_j below the pivot||^2;
+      // vals[1 + a] = v_a . v_{j-1} (a < j-1, the compact-WY Gram); vals[q] =
+      // S_q = x_j . x_q below the pivot (q > j, x_j not yet scaled).  The
+      // update's dots follow from them: with v = [1; x_j,below * scale],
+      // v . x_q = x_q[p] + scale * S_q, the pivot row x_q[p] broadcast with it.
       double vals[W];
 #pragma unroll
       for (int q = 0; q < W; ++q) vals[q] = 0.0;
@@ -96,7 +103,11 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
       for (int i = 0; i < R; ++i) {
         int64_t r = c0 + t + (int64_t)PT * i;
         if (r < m) {
-          if (r > p) vals[0] = fma(x[i][j], x[i][j], vals[0]);
+          if (r > p) {
+            vals[0] = fma(x[i][j], x[i][j], vals[0]);
+#pragma unroll
+            for (int q = j + 1; q < W; ++q) vals[q] = fma(x[i][j], x[i][q], vals[q]);
+          }
           if (j >= 2 && r > c0 + j - 1) {
 #pragma unroll
             for (int a = 0; a < j - 1; ++a) vals[1 + a] = fma(x[i][a], x[i][j - 1], vals[1 + a]);
@@ -108,8 +119,12 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
 #pragma unroll
         for (int a = 0; a < j - 1; ++a) vals[1 + a] += x[0][a];
       }
-      if (t == j) piv[j] = x[0][j];
-      block_sum<W>(vals, j >= 2 ? j : 1, lds, par);
+      if (t == j) {
+        piv[j] = x[0][j];
+#pragma unroll
+        for (int q = j + 1; q < W; ++q) prow[j][q] = x[0][q];
+      }
+      block_sum<W>(vals, W, lds, par);
       if (j >= 2 && t == 0) {
 #pragma unroll
         for (int a = 0; a < j - 1; ++a) gsh[a][j - 1] = vals[1 + a];
@@ -128,36 +143,22 @@ __global__ void __launch_bounds__(PT) qr_panel_kernel(double* __restrict__ A, in
         int64_t r = c0 + t + (int64_t)PT * i;
         if (r > p && r < m) x[i][j] *= scale;
       }
-      if (t == j) x[0][j] = beta;
       if (t == 0) taus[j] = tj;
       if (j + 1 < w && tj != 0.0) {
-        double d[W];
 #pragma unroll
-        for (int q = 0; q < W; ++q) d[q] = 0.0;
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-          int64_t r = c0 + t + (int64_t)PT * i;
-          if (r > p && r < m) {
-#pragma unroll
-            for (int q = j + 1; q < W; ++q) d[q] = fma(x[i][j], x[i][q], d[q]);
-          }
-        }
-        if (t == j) {
-#pragma unroll
-          for (int q = j + 1; q < W; ++q) d[q] += x[0][q];
-        }
-        block_sum<W>(d, w, lds, par);
+        for (int q = j + 1; q < W; ++q) vals[q] = tj * fma(scale, vals[q], prow[j][q]);
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           int64_t r = c0 + t + (int64_t)PT * i;
           if (r >= p && r < m) {
-            double v = (r == p) ? 1.0 : x[i][j];
+            const double v = (r == p) ? 1.0 : x[i][j];
 #pragma unroll
             for (int q = j + 1; q < W; ++q)
-              if (q < w) x[i][q] -= tj * d[q] * v;
+              if (q < w) x[i][q] = fma(-vals[q], v, x[i][q]);
           }
         }
       }
+      if (t == j) x[0][j] = beta;
     }
   }
   // Gram entries v_a . v_{w-1} of the last column (a < w-1)
