@@ -64,13 +64,27 @@ __global__ void __launch_bounds__(256) cq_pivmin(const double* __restrict__ S, i
 // Factor the w x w diagonal block at (j0, j0) of S (column-major, ld):
 // L11 (lower, zeros above) into F at the same position and X = L11^{-1}
 // (QB x QB, column-major).  512 threads: thread (q, t) keeps row t of the
-// block, columns QC q .. QC q + QC - 1, in registers, for both the Cholesky
-// elimination and X, which is accumulated by the same column sweep
-// (X <- I; at step c: X[c,:] /= L[c,c], X[t,:] -= L[t,c] X[c,:] for t > c).
-// One barrier per column.  A pivot not above *pivmin sets *fail and is
-// replaced by *pivmin (or 1) so that everything stays finite; the caller
-// discards the result.  Padding rows/columns (>= w) are zero with unit
-// pivots.
+// block, columns QC q .. QC q + QC - 1 (wave q's panel), in registers, for
+// both the Cholesky factor and X.  Blocked right-looking, one barrier per
+// panel:
+//   * wave b factors panel b alone: per column c the pivot and the panel's
+//     entries of column c come from other lanes by readlane, and the
+//     panel's later columns take the rank-1 update (no barrier); it
+//     publishes the panel's L columns and pivot reciprocals to LDS;
+//   * after the barrier, every later wave applies the panel's rank-8 update
+//     A[t][k] -= sum_c L[t][c] L[k][c] to its own columns (wave b + 1 then
+//     factors panel b + 1 at once), and every wave carries X's rows through
+//     the panel: the panel's rows of X are finalised by forward
+//     substitution with the panel's diagonal block (X_p <- L_pp^{-1} X_p,
+//     lanes 0..7, one per column of the wave), then the rows below take
+//     X[t] -= sum_c L[t][c] X_p[c].  Wave b + 1 does its X step for panel b
+//     one iteration later, after factoring panel b + 1, so that the chain of
+//     panel factorisations does not wait for it (every panel keeps its own
+//     LDS slot).
+// This is the elimination X <- I; X[c,:] /= L[c,c]; X[t,:] -= L[t,c] X[c,:]
+// (t > c) column by column, grouped by panel.  A pivot not above *pivmin
+// sets *fail and is replaced by *pivmin (or 1); the caller discards the
+// result.  Padding rows/columns (>= w) are zero with unit pivots.
 // DROP (the shifted PSD factorisation of tn_eigh_psd): *pivmin is the
 // shift delta; a pivot below delta / 2 (or not positive) drops its column
 // (L column and X row zero) instead of failing.
@@ -80,10 +94,12 @@ __global__ void __launch_bounds__(512) cq_diag(const double* __restrict__ S, int
                                                const double* __restrict__ pivmin,
                                                double* __restrict__ F, double* __restrict__ X,
                                                int* __restrict__ fail) {
-  // column c of the Schur complement and row c of X, unscaled, double
-  // buffered by the parity of c: one barrier per column
-  __shared__ double colb[2][QB];
-  __shared__ double xb[2][QB];
+  // panel columns of L (Ls[b][kc][t] = L[t][QC b + kc], zero above the
+  // diagonal) and 1 / L[c][c] (0 for a dropped column); XP: per wave, the
+  // panel's rows of X at its columns
+  __shared__ double Ls[QB / QC][QC][QB];
+  __shared__ double invs[QB / QC][QC];
+  __shared__ double XP[QB / QC][QC][QC + 1];
   const int t = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
   const double pm = DROP ? 0.5 * *pivmin : *pivmin;
@@ -95,55 +111,87 @@ __global__ void __launch_bounds__(512) cq_diag(const double* __restrict__ S, int
     r[k] = (t < w && c < w) ? S[(j0 + c) * ld + j0 + t] : 0.0;
     x[k] = c == t ? 1.0 : 0.0;
   }
-  // the column loop is unrolled by QC only (register indices static), so
-  // the loop body stays resident in the instruction cache
+  // wave b: factor panel b (its own columns) and publish it into slot pb
+  auto factor = [&](int b, int pb) {
+    const int c0 = QC * b;
+#pragma unroll
+    for (int kc = 0; kc < QC; ++kc) {
+      const int c = c0 + kc;
+      const double a = r[kc];
+      const double dv = readlane_f64(a, c);
+      double d, inv;
+      if (DROP) {
+        const bool keep = c >= w || (dv >= pm && dv > 0.0);
+        d = c < w ? (keep ? dv : 1.0) : 1.0;
+        inv = keep ? rsq_nr(d) : 0.0;
+      } else {
+        const bool ok = dv > pm;  // false for NaN
+        bad = bad || (c < w && !ok);
+        d = c < w ? (ok ? dv : (pm > 0.0 ? pm : 1.0)) : 1.0;
+        inv = rsq_nr(d);
+      }
+      const double l = t > c ? a * inv : 0.0;
+      const double co = l * inv;
+#pragma unroll
+      for (int k = kc + 1; k < QC; ++k) r[k] -= co * readlane_f64(a, c0 + k);
+      r[kc] = t == c ? d * inv : l;
+      Ls[pb][kc][t] = r[kc];
+      if (t == 0) invs[pb][kc] = inv;
+    }
+  };
+  // X through panel b at this wave's columns: the panel's rows finalised,
+  // then the rank-8 update of the rows below
+  auto xstep = [&](int b) {
+    const int c0 = QC * b;
+    double lt[QC];
+#pragma unroll
+    for (int kc = 0; kc < QC; ++kc) lt[kc] = Ls[b][kc][t];
+    if (t >= c0 && t < c0 + QC) {
+#pragma unroll
+      for (int k = 0; k < QC; ++k) XP[q][t - c0][k] = x[k];
+    }
+    wave_lds_sync();
+    if (t < QC) {
+      double s[QC];
+#pragma unroll
+      for (int i = 0; i < QC; ++i) s[i] = XP[q][i][t];
+#pragma unroll
+      for (int i = 0; i < QC; ++i) {
+#pragma unroll
+        for (int kk = 0; kk < i; ++kk) s[i] -= Ls[b][kk][c0 + i] * s[kk];
+        s[i] *= invs[b][i];
+      }
+#pragma unroll
+      for (int i = 0; i < QC; ++i) XP[q][i][t] = s[i];
+    }
+    wave_lds_sync();
+    const int ip = min(max(t - c0, 0), QC - 1);
+#pragma unroll
+    for (int k = 0; k < QC; ++k) {
+      double acc = x[k];
+#pragma unroll
+      for (int i = 0; i < QC; ++i) acc -= lt[i] * XP[q][i][k];
+      x[k] = t >= c0 + QC ? acc : (t >= c0 ? XP[q][ip][k] : x[k]);
+    }
+    wave_lds_sync();
+  };
+  if (q == 0) factor(0, 0);
 #pragma unroll 1
-  for (int qc = 0; qc < QB / QC; ++qc)
+  for (int b = 0; b < QB / QC; ++b) {
+    __syncthreads();  // panel b published
+    if (q > b) {
 #pragma unroll
-  for (int kc = 0; kc < QC; ++kc) {
-    const int c = QC * qc + kc, p = kc & 1;
-    if (q == qc) colb[p][t] = r[kc];
-    if (t == c) {
+      for (int kc = 0; kc < QC; ++kc) {
+        const double l = Ls[b][kc][t];
 #pragma unroll
-      for (int k = 0; k < QC; ++k) xb[p][QC * q + k] = x[k];
+        for (int k = 0; k < QC; ++k) r[k] -= l * Ls[b][kc][QC * q + k];
+      }
     }
-    __syncthreads();
-    // every LDS value of the step is read before any is used (no branches
-    // between the barrier and the loads: entries at or above the pivot are
-    // finite leftovers, masked by selects)
-    const double dv = colb[p][c], ct = colb[p][t];
-    double cl[QC], xr[QC];
-#pragma unroll
-    for (int k = 0; k < QC; ++k) {
-      cl[k] = colb[p][QC * q + k];
-      xr[k] = xb[p][QC * q + k];
-    }
-    double d, inv;
-    if (DROP) {
-      const bool keep = c >= w || (dv >= pm && dv > 0.0);
-      d = c < w ? (keep ? dv : 1.0) : 1.0;
-      inv = keep ? rsq_nr(d) : 0.0;
+    if (q == b + 1) {
+      factor(b + 1, b + 1);
     } else {
-      const bool ok = dv > pm;  // false for NaN
-      bad = bad || (c < w && !ok);
-      d = c < w ? (ok ? dv : (pm > 0.0 ? pm : 1.0)) : 1.0;
-      inv = rsq_nr(d);
-    }
-    const double l = t > c ? ct * inv : 0.0;
-    if (q == qc) {
-      if (t == c) r[kc] = d * inv;
-      if (t > c) r[kc] = l;
-    }
-    if (t == c) {
-#pragma unroll
-      for (int k = 0; k < QC; ++k) x[k] *= inv;
-    }
-    const double li = l * inv;
-#pragma unroll
-    for (int k = 0; k < QC; ++k) {
-      const double lk = QC * q + k > c ? cl[k] : 0.0;
-      r[k] -= li * lk;
-      x[k] -= li * xr[k];
+      if (q == b && b > 0) xstep(b - 1);
+      xstep(b);
     }
   }
 #pragma unroll
@@ -152,7 +200,7 @@ __global__ void __launch_bounds__(512) cq_diag(const double* __restrict__ S, int
     if (t < w && c < w) F[(j0 + c) * ld + j0 + t] = c <= t ? r[k] : 0.0;
     X[c * QB + t] = (t < w && c < w) ? x[k] : 0.0;
   }
-  if (!DROP && bad && threadIdx.x == 0) fail[0] = 1;
+  if (!DROP && bad && t == 0) fail[0] = 1;
 }
 
 // orth = max |G - I| over the n x n block G (column-major, ld); block c

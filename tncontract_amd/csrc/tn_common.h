@@ -137,6 +137,23 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// The FP64 value of lane `lane` (wave-uniform) in every lane.
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)u, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Orders a wave's LDS writes before its later LDS reads by other lanes of the
+// same wave: the wave issues its LDS instructions in order, and the fences
+// keep the compiler from moving accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Maximum over the wave of an FP32 value by DPP, as wave_sum_dpp (exact in
 // any order, so bit-identical to a shuffle reduction).
 template <int CTRL, int ROWMASK>
