@@ -408,6 +408,13 @@ def zeros_tensor(*args, **kwargs):
 
 
 # ---------------------------------------------------------------- contraction
+def _fuses(t, axes):
+    """True when the axes, in this order and size-1 axes aside, fuse into one
+    stride (each stride is the next one times the next extent)."""
+    ax = [a for a in axes if t.shape[a] != 1]
+    return all(t.stride(a) == t.stride(b) * t.shape[b] for a, b in zip(ax, ax[1:]))
+
+
 def _contract_data(d1, ax1, d2, ax2, precision="native"):
     """tensordot of two device tensors over axis lists ax1/ax2: output axes are
     the free axes of d1 then those of d2 (each in order)."""
@@ -427,6 +434,25 @@ def _contract_data(d1, ax1, d2, ax2, precision="native"):
     out = torch.empty(tuple(out_shape), dtype=dtype, device=d1.device)
     if out.numel() == 0:
         return out
+    M = int(np.prod(out_shape[:len(free1)])) if free1 else 1
+    N = int(np.prod(out_shape[len(free1):])) if free2 else 1
+    K = int(np.prod([d1.shape[a] for a in ax1])) if ax1 else 1
+    # An operand whose free or contracted axes do not fuse into one stride
+    # makes the MFMA contraction decode its tile offsets through per-mode
+    # tables (15-30 % slower on the sweep's products).  For a GEMM-sized
+    # product (every extent >= 64, >= 512 flops per operand element) one
+    # permuted dense copy in (free, contracted) / (contracted, free) order
+    # costs less, as the transpose inside np.tensordot (tensor.py:735) does.
+    if min(M, N, K) >= 64:
+        flops = 2.0 * M * N * K
+        if (not (_fuses(d1, free1) and _fuses(d1, ax1)) and d1.numel() >= 65536
+                and flops >= 512.0 * d1.numel()):
+            d1 = L.contiguous(d1.permute(*(free1 + ax1)))
+            free1, ax1 = list(range(len(free1))), list(range(len(free1), d1.dim()))
+        if (not (_fuses(d2, ax2) and _fuses(d2, free2)) and d2.numel() >= 65536
+                and flops >= 512.0 * d2.numel()):
+            d2 = L.contiguous(d2.permute(*(ax2 + free2)))
+            ax2, free2 = list(range(len(ax2))), list(range(len(ax2), d2.dim()))
     # dense output strides
     strides, acc = [0] * len(out_shape), 1
     for i in range(len(out_shape) - 1, -1, -1):
@@ -439,9 +465,6 @@ def _contract_data(d1, ax1, d2, ax2, precision="native"):
     bn = L.tensor_modes(d2, free2)
     cm = L.modes(out_shape[:nf1], strides[:nf1])
     cn = L.modes(out_shape[nf1:], strides[nf1:])
-    M = int(np.prod(out_shape[:nf1])) if nf1 else 1
-    N = int(np.prod(out_shape[nf1:])) if len(out_shape) > nf1 else 1
-    K = int(np.prod([d1.shape[a] for a in ax1])) if ax1 else 1
     L.contract(d1, am, ak, d2, bk, bn, out, cm, cn, L.dtype_code(d1), prec, M, N, K)
     return out
 
