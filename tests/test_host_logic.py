@@ -88,3 +88,20 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "--gpus 2 is not supported" in r.stderr
     assert r.stdout == ""
+
+
+def test_stride_fusion_predicate():
+    # decides whether a contraction operand is copied into a dense
+    # (free, contracted) order before the MFMA product
+    import torch
+    from tncontract_amd.tensor import _fuses
+    C = torch.zeros(4, 3, 5, 2)                 # dense (a, k, al, t)
+    assert _fuses(C, [0, 1, 2, 3]) and _fuses(C, [2, 3]) and _fuses(C, [1, 2])
+    assert not _fuses(C, [0, 2])                # a and al are not neighbours
+    assert not _fuses(C, [3, 1])                # t before k: strides do not nest
+    assert _fuses(C, [0]) and _fuses(C, [])
+    P = C.permute(0, 2, 1, 3)                   # (a, al, k, t) view of the same data
+    assert not _fuses(P, [0, 1])
+    assert _fuses(P.contiguous(), [0, 1])
+    U = torch.zeros(4, 1, 5)                    # size-1 axes do not break a fusion
+    assert _fuses(U, [0, 1, 2])
