@@ -164,3 +164,28 @@ def test_sumsq_and_scale(L):
     assert abs(L.sumsq(t) - np.sum(a * a)) < 1e-9 * np.sum(a * a)
     L.scale_(t, 0.5)
     np.testing.assert_allclose(t.cpu().numpy(), 0.5 * a)
+
+
+@pytest.mark.parametrize("prec,tol", [("native", 1e-12), ("fp32", 2e-5)])
+def test_contract_with_non_fusing_operands(L, prec, tol):
+    # environment-style products whose contracted axes are not neighbours in
+    # one operand (GEMM-sized, so the operand is first copied into dense
+    # (free, contracted) order): F = E(a,al,a',al') . C(a,k,al,t) over (a, al),
+    # then K = C . F over (a, al), with C on the left
+    from tncontract_amd import tensor as T
+    rng = np.random.default_rng(7)
+    E = rng.standard_normal((128, 5, 128, 5))
+    C = rng.standard_normal((128, 128, 5, 2))
+    F_ref = np.tensordot(E, C, axes=([0, 1], [0, 2]))
+    F = T._contract_data(dev(E), [0, 1], dev(C), [0, 2], prec)
+    scale = np.abs(F_ref).max()
+    assert F.shape == F_ref.shape
+    assert np.abs(F.double().cpu().numpy() - F_ref).max() <= tol * scale * 10
+    K_ref = np.tensordot(C, F_ref, axes=([0, 2], [0, 1]))
+    K = T._contract_data(dev(C), [0, 2], dev(F_ref), [0, 1], prec)
+    assert K.shape == K_ref.shape
+    assert np.abs(K.double().cpu().numpy() - K_ref).max() <= tol * np.abs(K_ref).max() * 10
+    # a strided (transposed) view as the operand
+    Ct = dev(C.transpose(1, 0, 3, 2).copy()).permute(1, 0, 3, 2)
+    F2 = T._contract_data(dev(E), [0, 1], Ct, [0, 2], prec)
+    assert np.abs(F2.double().cpu().numpy() - F_ref).max() <= tol * scale * 10

@@ -221,24 +221,27 @@ __global__ void __launch_bounds__(256) vec_norms(const double* __restrict__ Wv, 
 // NaN ranks as -inf so that the ranks stay a permutation of 0..k-1.
 __device__ __forceinline__ double rank_key(double x) { return x == x ? x : -INFINITY; }
 
-// 256 elements j per block of 1024 threads: the four lanes l, l + 16,
-// l + 32, l + 48 of a wave count over every fourth key of an LDS tile of
-// 2048 (interleaved, so each reads from LDS in step with the others) and
-// their counts are summed by shuffles; the comparisons are unrolled by
-// eight so that the LDS reads of a group are in flight together.
+// 64 elements j per block of 256 threads (k = 1024 spreads over 16 CUs;
+// 1024-thread blocks of 256 elements kept it on 4 and took 15.5 us): the
+// four lanes l, l + 16, l + 32, l + 48 of a wave count over every fourth key
+// of an LDS tile of 2048 (interleaved, so each reads from LDS in step with
+// the others) and their counts are summed by shuffles; the comparisons are
+// unrolled by eight so that the LDS reads of a group are in flight together.
 constexpr int RANK_TILE = 2048;
-__global__ void __launch_bounds__(1024) rank_sort(const double* __restrict__ sig, int64_t k,
-                                                  int* __restrict__ perm, double* __restrict__ S) {
+constexpr int RANK_NT = 256;
+__global__ void __launch_bounds__(RANK_NT) rank_sort(const double* __restrict__ sig, int64_t k,
+                                                     int* __restrict__ perm,
+                                                     double* __restrict__ S) {
   __shared__ double tile[RANK_TILE];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane >> 4;
-  const int64_t j = (int64_t)blockIdx.x * 256 + wave * 16 + (lane & 15);
+  const int64_t j = (int64_t)blockIdx.x * (RANK_NT / 4) + wave * 16 + (lane & 15);
   const double sj = j < k ? rank_key(sig[j]) : 0.0;
   int cnt = 0;
   for (int64_t t0 = 0; t0 < k; t0 += RANK_TILE) {
     const int tn = (int)min<int64_t>(RANK_TILE, k - t0);
     __syncthreads();
-    for (int i = threadIdx.x; i < tn; i += 1024) tile[i] = rank_key(sig[t0 + i]);
+    for (int i = threadIdx.x; i < tn; i += RANK_NT) tile[i] = rank_key(sig[t0 + i]);
     __syncthreads();
     // this lane's keys: i = q + 4 m
     const int64_t jr = j - t0;  // key i precedes j on ties when i < jr
@@ -265,8 +268,8 @@ __global__ void __launch_bounds__(1024) rank_sort(const double* __restrict__ sig
 }
 
 static void launch_rank_sort(const double* sig, int64_t k, int* perm, double* S, hipStream_t st) {
-  hipLaunchKernelGGL(rank_sort, dim3((unsigned)ceil_div(std::max<int64_t>(k, 1), 256)), dim3(1024),
-                     0, st, sig, k, perm, S);
+  hipLaunchKernelGGL(rank_sort, dim3((unsigned)ceil_div(std::max<int64_t>(k, 1), RANK_NT / 4)),
+                     dim3(RANK_NT), 0, st, sig, k, perm, S);
 }
 
 // U (m x k row-major) and Vt (k x n row-major).  Column j of U is vector
